@@ -1,0 +1,231 @@
+/*
+ * C ABI of the MI355X (gfx950) kernels of the coupled LBM-MPM time step.
+ *
+ * The reference solver (lizhuoh9/taichi-mpm-lbm) has no FFI: its boundary is
+ * the Python object API of fsi/lbm3d.py, fsi/mpm3d.py and fsi/coupling.py,
+ * whose @ti.kernel methods run on Taichi fields. Each entry point below
+ * replaces one or a fused group of those kernels; the comment names them.
+ * The Python package (fsi/_native.py) binds this header with ctypes and keeps
+ * the reference's class API on top.
+ *
+ * Conventions
+ *  - every pointer is a device pointer unless named host_*; sizes are element
+ *    counts; no entry point allocates, frees or synchronises unless it says so.
+ *  - every entry point enqueues on the given HIP stream and returns 0 on
+ *    success or a negative FSI_ERR_* code; fsi_last_error() then holds a
+ *    message for the calling thread. Nothing throws across the ABI.
+ *  - fields are SoA: component c of cell n lives at [c * ncells + n]; the cell
+ *    index is n = (i * ny + j) * nz + k, the reference's row-major order.
+ *  - `real` buffers are float for FSI_REAL_F32 and double for FSI_REAL_F64.
+ */
+#ifndef FSI_HIP_H_
+#define FSI_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSI_ABI_VERSION 1
+
+enum {
+  FSI_OK = 0,
+  FSI_ERR_ARGUMENT = -1,
+  FSI_ERR_HIP = -2,
+};
+
+enum { FSI_REAL_F32 = 0, FSI_REAL_F64 = 1 };
+
+/* population storage: absolute FP32 (reference), shifted+scaled FP16, FP64 */
+enum { FSI_POP_F32 = 0, FSI_POP_F16 = 1, FSI_POP_F64 = 2 };
+
+/* x-face boundary modes, fsi/lbm3d.py:16-27 */
+enum { FSI_BC_PERIODIC = 0, FSI_BC_PRESSURE = 1, FSI_BC_VELOCITY = 2 };
+
+/* -------------------------------------------------------------- LBM ---- */
+
+typedef struct fsi_lbm_params {
+  int32_t nx, ny, nz;
+  int32_t storage;         /* FSI_POP_* ; FSI_POP_F64 implies real = double */
+  double rho0;
+  double initial_velocity[3];
+  double s_v, s_o;         /* relaxation rates, fsi/lbm3d.py:141-168 */
+  int32_t left_mode, right_mode;
+  double bc_rho[2];        /* x_left.rho, x_right.rho */
+  double bc_vel[2][3];     /* x_left.velocity, x_right.velocity */
+  double pop_scale;        /* FP16 storage scale (power of two); ignored otherwise */
+} fsi_lbm_params;
+
+typedef struct fsi_lbm_buffers {
+  const int32_t* solid;    /* N, 0 fluid / 1 solid */
+  uint32_t* nbmask;        /* N, bit q set when the q-neighbour (bit 0: the cell) is solid */
+  void* rho;               /* N real: macroscopic density */
+  void* vel;               /* 3N real: macroscopic velocity */
+  void* force;             /* 3N real: per-cell body force (fsi/lbm3d.py:126) */
+  void* face_vel;          /* 2*3*ny*nz real: velocity of the x=0 / x=nx-1 cells used by the
+                              pressure boundary of the next step */
+} fsi_lbm_buffers;
+
+/* Force seen by one step. When `field` is null, every fluid cell uses
+ * `uniform` and no force memory is read. */
+typedef struct fsi_lbm_force {
+  const void* field;       /* 3N real or null */
+  double uniform[3];
+} fsi_lbm_force;
+
+/* fsi/lbm3d.py:193-203 + 219-267 (initialize: static_init, init_fields):
+ * neighbour masks, rho = rho0, u = u0 (0 on solid), both population
+ * buffers = feq(rho0, u). */
+int fsi_lbm_init(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pop_a, void* pop_b,
+                 void* stream);
+
+/* One LBM step, fsi/lbm3d.py:427-433 (collide 314-333, stream 352-361,
+ * apply_boundary 363-407), fused: each fluid cell reads its populations from
+ * pop_in, forms (rho, u) of the previous step (from the stored macroscopic
+ * fields when macro_in != 0, else from pop_in and `force`), collides with
+ * `force` and pushes to pop_out with half-way bounce-back; a second launch
+ * applies the x-face boundaries on pop_out. update_macro (409-425) is not
+ * run here: see fsi_lbm_macro. */
+int fsi_lbm_step(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop_in,
+                 void* pop_out, const fsi_lbm_force* force, int32_t macro_in, void* stream);
+
+/* fsi/lbm3d.py:409-425 (update_macro): rho and u from pop and force. */
+int fsi_lbm_macro(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop,
+                  const fsi_lbm_force* force, void* stream);
+
+/* fsi/lbm3d.py:499-534. op: 0 clear, 1 set (fluid = value, solid = 0; used by
+ * reset_force and set_uniform_force), 2 add (fluid += value, solid = 0),
+ * 3 zero on solid. `src` (3N real, may equal b->force) is read by op 2/3. */
+int fsi_lbm_force_op(const fsi_lbm_params* p, const fsi_lbm_buffers* b, int32_t op,
+                     const double* host_value, const void* src, void* dst, void* stream);
+
+/* fsi/lbm3d.py:475-478 (set_force_from_numpy): AoS (N,3) float32 -> SoA
+ * force, zero on solid. */
+int fsi_lbm_force_import(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const float* aos,
+                         void* stream);
+
+/* fsi/lbm3d.py:444-445 (distribution_numpy): populations as the reference's
+ * absolute values, AoS (N,19), real. */
+int fsi_lbm_export_populations(const fsi_lbm_params* p, const void* pop, void* out_aos,
+                               void* stream);
+
+/* SoA (ncomp, n) -> AoS (n, ncomp) transposition for the *_numpy accessors. */
+int fsi_soa_to_aos(int32_t real, int64_t n, int32_t ncomp, const void* soa, void* aos,
+                   void* stream);
+int fsi_aos_to_soa(int32_t real, int64_t n, int32_t ncomp, const void* aos, void* soa,
+                   void* stream);
+
+/* -------------------------------------------------------------- MPM ---- */
+
+typedef struct fsi_mpm_params {
+  int32_t nx, ny, nz;
+  int32_t real;            /* FSI_REAL_* */
+  int32_t num_particles;
+  int32_t capacity;        /* max_particles: stride of the particle SoA arrays */
+  double dx, inv_dx, dt;
+  double two_mu, lam;      /* 2*mu and lambda, fsi/mpm3d.py:33-38 */
+  double gravity[3];
+  int32_t boundary_width;
+  double boundary_damping;
+  int32_t apic;
+  double clamp_lower, clamp_upper[3];
+} fsi_mpm_params;
+
+typedef struct fsi_mpm_buffers {
+  void* x;                 /* 3*cap real */
+  void* v;                 /* 3*cap real */
+  void* C;                 /* 9*cap real, row-major 3x3 per particle */
+  void* F;                 /* 9*cap real */
+  const void* mass;        /* cap real */
+  const void* volume;      /* cap real */
+  const int32_t* active;   /* cap */
+  void* pforce;            /* 3*cap real */
+  void* grid;              /* 4N real, AoS (momentum/velocity xyz, mass) per node */
+} fsi_mpm_buffers;
+
+/* Coupling inputs/outputs read and written inside the particle passes
+ * (fsi/coupling.py:332-406 and 463-496). */
+typedef struct fsi_coupling_params {
+  int32_t enabled;         /* run the penalty coupling in the P2G pass */
+  int32_t build_fraction;  /* scatter the occupancy phi on this pass */
+  double gamma_eff;        /* gamma with the ramp applied (coupling.py:321-330) */
+  double dt_ratio;
+  double cell_volume;
+  double min_valid_weight;
+  int32_t use_relative_velocity_limit;
+  double relative_velocity_limit;
+  int32_t use_force_limit;
+  double force_limit;
+  int32_t contact_enabled; /* run the contact pass after G2P */
+  double contact_threshold;
+  double contact_damping;
+} fsi_coupling_params;
+
+typedef struct fsi_coupling_buffers {
+  const int32_t* solid;    /* N lattice solid mask */
+  const void* lbm_vel;     /* 3N real, macroscopic velocity of the lattice */
+  void* cforce;            /* 4N real, AoS (coupling force density xyz, phi) per cell */
+  void* valid_weight;      /* cap real */
+  int32_t* coupling_mask;  /* cap */
+  int32_t* contact_mask;   /* cap */
+  int32_t* counters;       /* 8: unsupported, partial, clipped, contact, damped, ib active,
+                              ib clipped, spare */
+} fsi_coupling_buffers;
+
+/* Zero the grid (fsi/mpm3d.py:205-209, clear_grid). */
+int fsi_mpm_clear_grid(const fsi_mpm_params* p, const fsi_mpm_buffers* b, void* stream);
+
+/* Particle-to-grid pass, fsi/mpm3d.py:234-266, with the penalty coupling of
+ * fsi/coupling.py:332-406 fused in when c->enabled (velocity interpolation,
+ * force, reaction and phi scatter) and the per-substep particle force clear
+ * of fsi/mpm3d.py:211-214 when clear_force != 0. do_p2g = 0 runs only the
+ * coupling part (LBMMpmCoupler.compute_coupling_forces). */
+int fsi_mpm_p2g(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+                const fsi_coupling_buffers* cb, int32_t clear_force, int32_t do_p2g, void* stream);
+
+/* fsi/mpm3d.py:268-288 (grid_op) then 290-328 (g2p + clamp), with the contact
+ * pass of fsi/coupling.py:463-496 fused after the position update when
+ * c->contact_enabled. */
+int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
+                     const fsi_coupling_params* c, const fsi_coupling_buffers* cb, void* stream);
+
+/* fsi/coupling.py:463-496 alone (used by compute_coupling_forces). */
+int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+                const fsi_coupling_buffers* cb, void* stream);
+
+/* Rotation factor R = U V^T of F = U S V^T per particle (fsi/mpm3d.py:245-246)
+ * for n matrices; exposed for tests of the in-kernel SVD. */
+int fsi_polar_rotation(int32_t real, int32_t n, const void* F, void* R, void* stream);
+
+/* --------------------------------------------------------- coupling ---- */
+
+typedef struct fsi_force_assembly {
+  int32_t coupling;        /* add the coupling force (coupling.py:413-419) */
+  int32_t clamp_fraction;  /* phi = min(phi, 1) (coupling.py:408-411) */
+  int32_t ib_enabled;      /* immersed-boundary drag (coupling.py:421-461) */
+  double ib_drag, ib_threshold, ib_max_force;
+  int32_t ib_use_max;
+  double base_force[3];    /* LBMConfig.force: the value of reset_force */
+} fsi_force_assembly;
+
+/* LBMMpmCoupler.step tail, fsi/coupling.py:162-170: clamp phi, reset the
+ * lattice force, add the coupling force, compute and add the IB drag, all in
+ * one pass over the cells. ib_force (3N real) and ib_total (3 real, summed
+ * with atomics) may be null when IB is off. */
+int fsi_force_assemble(const fsi_lbm_params* lp, const fsi_lbm_buffers* lb,
+                       const fsi_force_assembly* a, void* cforce, void* ib_force, void* ib_total,
+                       int32_t* counters, int32_t write_force, void* stream);
+
+/* ----------------------------------------------------------- common ---- */
+
+int fsi_abi_version(void);
+const char* fsi_last_error(void);
+/* fills a real buffer with zeros (a kernel, usable inside stream capture) */
+int fsi_fill_zero(int64_t nbytes, void* dst, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSI_HIP_H_ */

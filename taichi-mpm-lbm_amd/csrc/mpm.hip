@@ -1,0 +1,783 @@
+// MLS-MPM particle passes with the penalty coupling fused in, for gfx950.
+//
+// Particles are SoA planes of `capacity` entries (x[a*cap + p], F[(r*3+s)*cap + p]),
+// so a wavefront loads 64 consecutive values per component. The grid is AoS
+// (momentum xyz, mass) per node, so the 4 atomics of one node hit one cache
+// line; the coupling buffer uses the same (force density xyz, phi) layout.
+//
+// Per substep: clear grid -> P2G (+ coupling) -> grid update + G2P (+ contact).
+
+#include "common.h"
+
+namespace fsi {
+namespace {
+
+template <typename R> struct MpmK {
+  int nx, ny, nz;
+  int num;
+  int cap;
+  R dx, inv_dx, dt;
+  R two_mu, lam;
+  R g[3];
+  int bw;
+  R damping;
+  int apic;
+  R four_inv_dx;
+  R lower, upper[3];
+};
+
+template <typename R> MpmK<R> make_mpm(const fsi_mpm_params* p) {
+  MpmK<R> m;
+  m.nx = p->nx;
+  m.ny = p->ny;
+  m.nz = p->nz;
+  m.num = p->num_particles;
+  m.cap = p->capacity;
+  m.dx = R(p->dx);
+  m.inv_dx = R(p->inv_dx);
+  m.dt = R(p->dt);
+  m.two_mu = R(p->two_mu);
+  m.lam = R(p->lam);
+  for (int a = 0; a < 3; ++a) m.g[a] = R(p->gravity[a]);
+  m.bw = p->boundary_width;
+  m.damping = R(p->boundary_damping);
+  m.apic = p->apic;
+  m.four_inv_dx = R(4.0 * p->inv_dx);
+  m.lower = R(p->clamp_lower);
+  for (int a = 0; a < 3; ++a) m.upper[a] = R(p->clamp_upper[a]);
+  return m;
+}
+
+template <typename R> struct CplK {
+  int enabled, build;
+  R gamma, dt_ratio, cell_volume, min_w;
+  int use_rel;
+  R rel_limit;
+  int use_force;
+  R force_limit;
+  int contact;
+  R contact_thr, contact_damp;
+};
+
+template <typename R> CplK<R> make_cpl(const fsi_coupling_params* c) {
+  CplK<R> k{};
+  if (!c) return k;
+  k.enabled = c->enabled;
+  k.build = c->build_fraction;
+  k.gamma = R(c->gamma_eff);
+  k.dt_ratio = R(c->dt_ratio);
+  k.cell_volume = R(c->cell_volume);
+  k.min_w = R(c->min_valid_weight);
+  k.use_rel = c->use_relative_velocity_limit;
+  k.rel_limit = R(c->relative_velocity_limit);
+  k.use_force = c->use_force_limit;
+  k.force_limit = R(c->force_limit);
+  k.contact = c->contact_enabled;
+  k.contact_thr = R(c->contact_threshold);
+  k.contact_damp = R(c->contact_damping);
+  return k;
+}
+
+// Adds the number of lanes with `pred` to *ctr with one atomic per wavefront.
+__device__ __forceinline__ void wave_count(int32_t* ctr, bool pred) {
+  const unsigned long long b = __ballot(pred);
+  const unsigned long long act = __ballot(1);
+  const int leader = __ffsll((long long)act) - 1;
+  if ((int)__lane_id() == leader && b) atomicAdd(ctr, (int)__popcll(b));
+}
+
+// fsi/mpm3d.py:227-232, quadratic B-spline weights per axis
+template <typename R> __device__ __forceinline__ void bspline(const R fx[3], R w[3][3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const R t0 = R(1.5) - fx[a];
+    const R t1 = fx[a] - R(1);
+    const R t2 = fx[a] - R(0.5);
+    w[0][a] = R(0.5) * (t0 * t0);
+    w[1][a] = R(0.75) - t1 * t1;
+    w[2][a] = R(0.5) * (t2 * t2);
+  }
+}
+
+// base node (truncating cast, fsi/mpm3d.py:239) and fractional offset
+// No contraction: x*inv_dx is rounded before the -0.5, exactly as written, so
+// the cell key of a particle on a cell face is the oracle's bit for bit.
+template <typename R>
+__device__ __forceinline__ void locate(const R x[3], R inv_dx, int base[3], R fx[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const R gp = x[a] * inv_dx;
+    base[a] = (int)(gp - R(0.5));
+    fx[a] = gp - R(base[a]);
+  }
+}
+
+__device__ __forceinline__ bool inside(int nx, int ny, int nz, int i, int j, int k) {
+  return (unsigned)i < (unsigned)nx && (unsigned)j < (unsigned)ny && (unsigned)k < (unsigned)nz;
+}
+
+// ------------------------------------------------------------- SVD ----
+//
+// Rotation factor R = U V^T of the rotation-variant SVD F = U S V^T (U, V
+// proper rotations), the quantity fsi/mpm3d.py:245-246 builds from ti.svd:
+// cyclic Jacobi on the symmetric F^T F gives V, a Givens QR of F V gives U
+// (its diagonal carries the singular values, the last with the sign of det F).
+
+template <typename R>
+__device__ __forceinline__ void jacobi_rotate(R A[3][3], R V[3][3], int p, int q) {
+  const R apq = A[p][q];
+  if (apq == R(0)) return;
+  const R theta = (A[q][q] - A[p][p]) / (R(2) * apq);
+  const R at = fabs(theta);
+  R t;
+  if (at > R(1e15))
+    t = R(0.5) / theta;
+  else
+    t = (theta >= R(0) ? R(1) : R(-1)) / (at + sqrt(theta * theta + R(1)));
+  const R c = R(1) / sqrt(t * t + R(1));
+  const R s = t * c;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const R akp = A[k][p], akq = A[k][q];
+    A[k][p] = c * akp - s * akq;
+    A[k][q] = s * akp + c * akq;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const R apk = A[p][k], aqk = A[q][k];
+    A[p][k] = c * apk - s * aqk;
+    A[q][k] = s * apk + c * aqk;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const R vkp = V[k][p], vkq = V[k][q];
+    V[k][p] = c * vkp - s * vkq;
+    V[k][q] = s * vkp + c * vkq;
+  }
+}
+
+template <typename R> __device__ __forceinline__ void swap_cols(R V[3][3], R lam[3], int a, int b) {
+  const R t = lam[a];
+  lam[a] = lam[b];
+  lam[b] = t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const R v = V[k][a];
+    V[k][a] = V[k][b];
+    V[k][b] = -v;  // keep det V = +1
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void rotation_core(const R F[9], R Rot[9], R* lam_min_over_max) {
+  R A[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+      A[r][s] = F[0 * 3 + r] * F[0 * 3 + s] + F[1 * 3 + r] * F[1 * 3 + s] + F[2 * 3 + r] * F[2 * 3 + s];
+  R V[3][3] = {{R(1), R(0), R(0)}, {R(0), R(1), R(0)}, {R(0), R(0), R(1)}};
+  constexpr int kSweeps = sizeof(R) == 8 ? 12 : 8;
+  const R eps = sizeof(R) == 8 ? R(1e-30) : R(1e-14);
+  for (int sweep = 0; sweep < kSweeps; ++sweep) {
+    const R off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+    const R dg = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+    if (off <= eps * dg) break;
+    jacobi_rotate(A, V, 0, 1);
+    jacobi_rotate(A, V, 0, 2);
+    jacobi_rotate(A, V, 1, 2);
+  }
+  R lam[3] = {A[0][0], A[1][1], A[2][2]};
+  // sort descending (three compare-swaps)
+  if (lam[0] < lam[1]) swap_cols(V, lam, 0, 1);
+  if (lam[0] < lam[2]) swap_cols(V, lam, 0, 2);
+  if (lam[1] < lam[2]) swap_cols(V, lam, 1, 2);
+  if (lam_min_over_max) *lam_min_over_max = lam[0] > R(0) ? lam[2] / lam[0] : R(0);
+  // B = F V and its Givens QR; Q accumulates the rotations
+  R B[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+      B[r][s] = F[r * 3 + 0] * V[0][s] + F[r * 3 + 1] * V[1][s] + F[r * 3 + 2] * V[2][s];
+  R Qm[3][3] = {{R(1), R(0), R(0)}, {R(0), R(1), R(0)}, {R(0), R(0), R(1)}};
+  constexpr int ra[3] = {0, 0, 1}, rb[3] = {1, 2, 2}, cc[3] = {0, 0, 1};
+#pragma unroll
+  for (int gi = 0; gi < 3; ++gi) {
+    const int a = ra[gi], b = rb[gi], col = cc[gi];
+    const R x = B[a][col], y = B[b][col];
+    const R r = sqrt(x * x + y * y);
+    R c = R(1), s = R(0);
+    if (r > R(0)) {
+      c = x / r;
+      s = y / r;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const R ba = B[a][k], bb = B[b][k];
+      B[a][k] = c * ba + s * bb;
+      B[b][k] = -s * ba + c * bb;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const R qa = Qm[k][a], qb = Qm[k][b];
+      Qm[k][a] = c * qa + s * qb;
+      Qm[k][b] = -s * qa + c * qb;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+      Rot[r * 3 + s] = Qm[r][0] * V[s][0] + Qm[r][1] * V[s][1] + Qm[r][2] * V[s][2];
+}
+
+// FP32 rotation with an FP64 recomputation for near-degenerate F: when the
+// squared singular values span more than 1e5 (F^T F loses the small one in
+// FP32) or F is inverted.
+template <typename R> __device__ __forceinline__ void rotation_factor(const R F[9], R Rot[9], R J) {
+  if constexpr (sizeof(R) == 8) {
+    rotation_core<R>(F, Rot, nullptr);
+  } else {
+    R ratio;
+    rotation_core<R>(F, Rot, &ratio);
+    if (!(ratio > R(1e-5)) || !(J > R(0))) {
+      double Fd[9], Rd[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Fd[i] = double(F[i]);
+      rotation_core<double>(Fd, Rd, nullptr);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Rot[i] = R(Rd[i]);
+    }
+  }
+}
+
+template <typename R> __device__ __forceinline__ R det3(const R m[9]) {
+  return m[0] * (m[4] * m[8] - m[7] * m[5]) - m[3] * (m[1] * m[8] - m[7] * m[2]) +
+         m[6] * (m[1] * m[5] - m[4] * m[2]);
+}
+
+// ------------------------------------------------------- P2G pass ----
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) buf[t] = R(0);
+}
+
+// fsi/coupling.py:332-406 (COUPLE) fused with fsi/mpm3d.py:234-266 (P2G).
+template <typename R, bool COUPLE, bool P2G>
+__global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
+                                                R* __restrict__ v, const R* __restrict__ C,
+                                                R* __restrict__ Fd, const R* __restrict__ mass,
+                                                const R* __restrict__ vol,
+                                                const int32_t* __restrict__ active,
+                                                R* __restrict__ pforce, R* __restrict__ grid,
+                                                const int32_t* __restrict__ solid,
+                                                const R* __restrict__ lbm_vel, R* __restrict__ cforce,
+                                                R* __restrict__ valid_weight,
+                                                int32_t* __restrict__ cmask,
+                                                int32_t* __restrict__ counters, int clear_force) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < m.num;
+  const bool act = live && active[p] != 0;
+  const int cap = m.cap;
+  const int64_t N = (int64_t)m.nx * m.ny * m.nz;
+
+  bool partial = false, unsupported = false, clipped = false;
+  R pf[3] = {R(0), R(0), R(0)};
+  R xp[3], vp[3], fx[3], w[3][3];
+  int base[3];
+  if (act) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      xp[a] = x[a * cap + p];
+      vp[a] = v[a * cap + p];
+    }
+    locate(xp, m.inv_dx, base, fx);
+    bspline(fx, w);
+  }
+
+  if constexpr (COUPLE) {
+    if (live && !act) {
+      valid_weight[p] = R(0);
+      cmask[p] = 0;
+    }
+    if (act) {
+      R wsum = R(0), u[3] = {R(0), R(0), R(0)};
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+            if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+            const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+            if (solid[node] != 0) continue;
+            const R wt = w[i][0] * w[j][1] * w[k][2];
+            wsum += wt;
+            u[0] += wt * lbm_vel[node];
+            u[1] += wt * lbm_vel[N + node];
+            u[2] += wt * lbm_vel[2 * N + node];
+          }
+      valid_weight[p] = wsum;
+      if (wsum > cp.min_w) {
+        partial = wsum < R(1) - R(1e-5);
+        R dv[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) dv[a] = u[a] / wsum - vp[a];
+        if (cp.use_rel) {
+          const R nrm = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+          if (nrm > cp.rel_limit && nrm > R(0)) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) dv[a] = dv[a] / nrm * cp.rel_limit;
+            clipped = true;
+          }
+        }
+        const R gm = cp.gamma * mass[p];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) pf[a] = gm * dv[a];
+        if (cp.use_force) {
+          const R nrm = sqrt(pf[0] * pf[0] + pf[1] * pf[1] + pf[2] * pf[2]);
+          if (nrm > cp.force_limit && nrm > R(0)) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pf[a] = pf[a] / nrm * cp.force_limit;
+            clipped = true;
+          }
+        }
+        cmask[p] = 1;
+        // equal and opposite reaction (and occupancy) spread on the valid nodes
+        const R fscale = cp.dt_ratio / cp.cell_volume;
+        const R phi = vol[p] / cp.cell_volume;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+              if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+              const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+              if (solid[node] != 0) continue;
+              const R nw = (w[i][0] * w[j][1] * w[k][2]) / wsum;
+              R* cf = cforce + 4 * node;
+              atomicAdd(cf + 0, -nw * pf[0] * fscale);
+              atomicAdd(cf + 1, -nw * pf[1] * fscale);
+              atomicAdd(cf + 2, -nw * pf[2] * fscale);
+              if (cp.build) atomicAdd(cf + 3, nw * phi);
+            }
+      } else {
+        unsupported = true;
+        cmask[p] = 0;
+      }
+    }
+    wave_count(counters + 0, unsupported);
+    wave_count(counters + 1, partial);
+    wave_count(counters + 2, clipped);
+    if (live) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) pforce[a * cap + p] = pf[a];
+    }
+  } else {
+    if (clear_force) {
+      if (live) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) pforce[a * cap + p] = R(0);
+      }
+    } else if (act) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) pf[a] = pforce[a * cap + p];
+    }
+  }
+
+  if constexpr (P2G) {
+    if (!act) return;
+    const R dt = m.dt;
+    R Cm[9], Fo[9], Fn[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      Cm[r] = C[r * cap + p];
+      Fo[r] = Fd[r * cap + p];
+    }
+    // F <- (I + dt C) F
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        R acc = R(0);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) acc += ((r == t ? R(1) : R(0)) + dt * Cm[r * 3 + t]) * Fo[t * 3 + s];
+        Fn[r * 3 + s] = acc;
+      }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) Fd[r * cap + p] = Fn[r];
+    const R J = det3(Fn);
+    R Rot[9];
+    rotation_factor<R>(Fn, Rot, J);
+    // P = 2 mu (F - R) F^T + lam J (J - 1) I; affine = -dt vol 4/dx^2 P + m C
+    const R scale = -dt * vol[p] * R(4) * m.inv_dx * m.inv_dx;
+    const R lj = m.lam * J * (J - R(1));
+    const R mp = mass[p];
+    R aff[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        R st = R(0);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) st += (m.two_mu * (Fn[r * 3 + t] - Rot[r * 3 + t])) * Fn[s * 3 + t];
+        if (r == s) st += lj;
+        aff[r * 3 + s] = scale * st + mp * Cm[r * 3 + s];
+      }
+    R mom0[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) mom0[a] = mp * vp[a];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+          if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+          const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+          const R dp[3] = {(R(i) - fx[0]) * m.dx, (R(j) - fx[1]) * m.dx, (R(k) - fx[2]) * m.dx};
+          const R wt = w[i][0] * w[j][1] * w[k][2];
+          R* g = grid + 4 * node;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const R mom = mom0[a] + (aff[a * 3 + 0] * dp[0] + aff[a * 3 + 1] * dp[1] + aff[a * 3 + 2] * dp[2]) +
+                          dt * pf[a];
+            atomicAdd(g + a, wt * mom);
+          }
+          atomicAdd(g + 3, wt * mp);
+        }
+  }
+}
+
+// ------------------------------------------------- grid + G2P pass ----
+
+// fsi/mpm3d.py:268-288
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_grid_op(MpmK<R> m, R* __restrict__ grid) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t N = (int64_t)m.nx * m.ny * m.nz;
+  if (node >= N) return;
+  R* g = grid + 4 * node;
+  const R mass = g[3];
+  if (!(mass > R(0))) return;
+  const int k = (int)(node % m.nz);
+  const int64_t ij = node / m.nz;
+  const int j = (int)(ij % m.ny);
+  const int i = (int)(ij / m.ny);
+  const int idx[3] = {i, j, k};
+  const int n[3] = {m.nx, m.ny, m.nz};
+  R vv[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) vv[a] = g[a] / mass + m.dt * m.g[a];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (idx[a] < m.bw && vv[a] < R(0)) vv[a] = -m.damping * vv[a];
+    if (idx[a] >= n[a] - m.bw && vv[a] > R(0)) vv[a] = -m.damping * vv[a];
+    g[a] = vv[a];
+  }
+}
+
+// fsi/coupling.py:463-496 for one particle at position xp
+template <typename R>
+__device__ __forceinline__ bool contact_test(const MpmK<R>& m, const CplK<R>& cp, const R xp[3],
+                                             const int32_t* __restrict__ solid,
+                                             const R* __restrict__ cforce) {
+  int base[3];
+  R fx[3], w[3][3];
+  locate(xp, m.inv_dx, base, fx);
+  bspline(fx, w);
+  R dyn = R(0), stat = R(0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+        if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+        const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+        const R wt = w[i][0] * w[j][1] * w[k][2];
+        dyn += wt * cforce[4 * node + 3];
+        if (solid[node] != 0) stat += wt;
+      }
+  return dyn >= cp.contact_thr || stat >= cp.contact_thr;
+}
+
+// fsi/mpm3d.py:290-328 (+ contact of fsi/coupling.py:463-496 when CONTACT)
+template <typename R, bool CONTACT>
+__global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
+                                                R* __restrict__ v, R* __restrict__ C,
+                                                const int32_t* __restrict__ active,
+                                                const R* __restrict__ grid,
+                                                const int32_t* __restrict__ solid,
+                                                const R* __restrict__ cforce,
+                                                int32_t* __restrict__ contact_mask,
+                                                int32_t* __restrict__ counters) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < m.num;
+  const bool act = live && active[p] != 0;
+  const int cap = m.cap;
+  bool cand = false, damped = false;
+  if (act) {
+    R xp[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xp[a] = x[a * cap + p];
+    int base[3];
+    R fx[3], w[3][3];
+    locate(xp, m.inv_dx, base, fx);
+    bspline(fx, w);
+    R nv[3] = {R(0), R(0), R(0)};
+    R nc[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+          if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+          const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+          const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
+          const R wt = w[i][0] * w[j][1] * w[k][2];
+          const R* g = grid + 4 * node;
+          const R gv[3] = {g[0], g[1], g[2]};
+#pragma unroll
+          for (int a = 0; a < 3; ++a) nv[a] += wt * gv[a];
+          const R s = m.four_inv_dx * wt;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
+        }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) C[r * cap + p] = m.apic ? nc[r] : R(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const R xn = xp[a] + m.dt * nv[a];
+      xp[a] = fmin(fmax(xn, m.lower), m.upper[a]);
+      x[a * cap + p] = xp[a];
+    }
+    if constexpr (CONTACT) {
+      cand = contact_test(m, cp, xp, solid, cforce);
+      if (cand && cp.contact_damp > R(0)) {
+        damped = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) nv[a] *= R(1) - cp.contact_damp;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a * cap + p] = nv[a];
+  }
+  if constexpr (CONTACT) {
+    if (live) contact_mask[p] = cand ? 1 : 0;
+    wave_count(counters + 3, cand);
+    wave_count(counters + 4, damped);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_contact(MpmK<R> m, CplK<R> cp, const R* __restrict__ x,
+                                                    R* __restrict__ v,
+                                                    const int32_t* __restrict__ active,
+                                                    const int32_t* __restrict__ solid,
+                                                    const R* __restrict__ cforce,
+                                                    int32_t* __restrict__ contact_mask,
+                                                    int32_t* __restrict__ counters) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p < m.num;
+  const bool act = live && active[p] != 0;
+  bool cand = false, damped = false;
+  if (act) {
+    const R xp[3] = {x[p], x[m.cap + p], x[2 * m.cap + p]};
+    cand = contact_test(m, cp, xp, solid, cforce);
+    if (cand && cp.contact_damp > R(0)) {
+      damped = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[a * m.cap + p] *= R(1) - cp.contact_damp;
+    }
+  }
+  if (live) contact_mask[p] = cand ? 1 : 0;
+  wave_count(counters + 3, cand);
+  wave_count(counters + 4, damped);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_polar(int n, const R* __restrict__ F, R* __restrict__ Rot) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  R f[9], r[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) f[i] = F[t * 9 + i];
+  rotation_factor<R>(f, r, det3(f));
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Rot[t * 9 + i] = r[i];
+}
+
+__global__ void k_zero_counters(int32_t* __restrict__ c, int first, int count) {
+  const int t = threadIdx.x;
+  if (t < count) c[first + t] = 0;
+}
+
+int check_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
+  FSI_REQUIRE(p && b, "mpm: null params or buffers");
+  FSI_REQUIRE(p->nx > 0 && p->ny > 0 && p->nz > 0, "mpm: bad grid dims");
+  FSI_REQUIRE(p->num_particles >= 0 && p->num_particles <= p->capacity,
+              "mpm: %d particles exceed capacity %d", p->num_particles, p->capacity);
+  FSI_REQUIRE(p->real == FSI_REAL_F32 || p->real == FSI_REAL_F64, "mpm: unknown real type");
+  return 0;
+}
+
+template <typename R>
+int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+          const fsi_coupling_buffers* cb, int clear_force, int do_p2g, hipStream_t st) {
+  MpmK<R> m = make_mpm<R>(p);
+  CplK<R> cp = make_cpl<R>(c);
+  const bool couple = c && c->enabled;
+  if (couple) {
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 0, 5);
+    FSI_TRY_LAUNCH("zero_counters");
+  }
+  if (m.num == 0) return 0;
+  const dim3 grid(blocks_for(m.num)), block(kBlock);
+#define FSI_P2G_ARGS                                                                            \
+  m, cp, (R*)b->x, (R*)b->v, (const R*)b->C, (R*)b->F, (const R*)b->mass, (const R*)b->volume, \
+      b->active, (R*)b->pforce, (R*)b->grid, cb ? cb->solid : nullptr,                          \
+      cb ? (const R*)cb->lbm_vel : nullptr, cb ? (R*)cb->cforce : nullptr,                      \
+      cb ? (R*)cb->valid_weight : nullptr, cb ? cb->coupling_mask : nullptr,                    \
+      cb ? cb->counters : nullptr, clear_force
+  if (couple && do_p2g)
+    hipLaunchKernelGGL((k_p2g<R, true, true>), grid, block, 0, st, FSI_P2G_ARGS);
+  else if (couple)
+    hipLaunchKernelGGL((k_p2g<R, true, false>), grid, block, 0, st, FSI_P2G_ARGS);
+  else if (do_p2g)
+    hipLaunchKernelGGL((k_p2g<R, false, true>), grid, block, 0, st, FSI_P2G_ARGS);
+  else
+    hipLaunchKernelGGL((k_p2g<R, false, false>), grid, block, 0, st, FSI_P2G_ARGS);
+#undef FSI_P2G_ARGS
+  FSI_TRY_LAUNCH("mpm_p2g");
+  return 0;
+}
+
+template <typename R>
+int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+               const fsi_coupling_buffers* cb, hipStream_t st) {
+  MpmK<R> m = make_mpm<R>(p);
+  CplK<R> cp = make_cpl<R>(c);
+  const int64_t N = (int64_t)m.nx * m.ny * m.nz;
+  hipLaunchKernelGGL(k_grid_op<R>, dim3(blocks_for(N)), dim3(kBlock), 0, st, m, (R*)b->grid);
+  FSI_TRY_LAUNCH("mpm_grid_op");
+  const bool contact = c && c->contact_enabled;
+  if (contact) {
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 3, 2);
+    FSI_TRY_LAUNCH("zero_counters");
+  }
+  if (m.num == 0) return 0;
+  if (contact)
+    hipLaunchKernelGGL((k_g2p<R, true>), dim3(blocks_for(m.num)), dim3(kBlock), 0, st, m, cp,
+                       (R*)b->x, (R*)b->v, (R*)b->C, b->active, (const R*)b->grid, cb->solid,
+                       (const R*)cb->cforce, cb->contact_mask, cb->counters);
+  else
+    hipLaunchKernelGGL((k_g2p<R, false>), dim3(blocks_for(m.num)), dim3(kBlock), 0, st, m, cp,
+                       (R*)b->x, (R*)b->v, (R*)b->C, b->active, (const R*)b->grid,
+                       (const int32_t*)nullptr, (const R*)nullptr, (int32_t*)nullptr,
+                       (int32_t*)nullptr);
+  FSI_TRY_LAUNCH("mpm_g2p");
+  return 0;
+}
+
+template <typename R>
+int contact_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+              const fsi_coupling_buffers* cb, hipStream_t st) {
+  MpmK<R> m = make_mpm<R>(p);
+  CplK<R> cp = make_cpl<R>(c);
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 3, 2);
+  FSI_TRY_LAUNCH("zero_counters");
+  if (m.num == 0) return 0;
+  hipLaunchKernelGGL(k_contact<R>, dim3(blocks_for(m.num)), dim3(kBlock), 0, st, m, cp,
+                     (const R*)b->x, (R*)b->v, b->active, cb->solid, (const R*)cb->cforce,
+                     cb->contact_mask, cb->counters);
+  FSI_TRY_LAUNCH("contact");
+  return 0;
+}
+
+}  // namespace
+}  // namespace fsi
+
+using namespace fsi;
+
+extern "C" {
+
+int fsi_mpm_clear_grid(const fsi_mpm_params* p, const fsi_mpm_buffers* b, void* stream) {
+  if (int rc = check_mpm(p, b)) return rc;
+  FSI_REQUIRE(b->grid, "mpm_clear_grid: null grid");
+  const int64_t n = 4LL * p->nx * p->ny * p->nz;
+  hipStream_t st = (hipStream_t)stream;
+  if (p->real == FSI_REAL_F64)
+    hipLaunchKernelGGL(k_clear<double>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, (double*)b->grid);
+  else
+    hipLaunchKernelGGL(k_clear<float>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, (float*)b->grid);
+  FSI_TRY_LAUNCH("mpm_clear_grid");
+  return 0;
+}
+
+int fsi_mpm_p2g(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+                const fsi_coupling_buffers* cb, int32_t clear_force, int32_t do_p2g, void* stream) {
+  if (int rc = check_mpm(p, b)) return rc;
+  FSI_REQUIRE(b->x && b->v && b->C && b->F && b->mass && b->volume && b->active && b->pforce,
+              "mpm_p2g: null particle buffer");
+  FSI_REQUIRE(!do_p2g || b->grid, "mpm_p2g: null grid");
+  if (c && c->enabled)
+    FSI_REQUIRE(cb && cb->solid && cb->lbm_vel && cb->cforce && cb->valid_weight &&
+                    cb->coupling_mask && cb->counters,
+                "mpm_p2g: coupling needs all coupling buffers");
+  hipStream_t st = (hipStream_t)stream;
+  if (p->real == FSI_REAL_F64) return p2g_t<double>(p, b, c, cb, clear_force, do_p2g, st);
+  return p2g_t<float>(p, b, c, cb, clear_force, do_p2g, st);
+}
+
+int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
+                     const fsi_coupling_params* c, const fsi_coupling_buffers* cb, void* stream) {
+  if (int rc = check_mpm(p, b)) return rc;
+  FSI_REQUIRE(b->x && b->v && b->C && b->active && b->grid, "mpm_grid_g2p: null buffer");
+  if (c && c->contact_enabled)
+    FSI_REQUIRE(cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters,
+                "mpm_grid_g2p: contact needs the coupling buffers");
+  hipStream_t st = (hipStream_t)stream;
+  if (p->real == FSI_REAL_F64) return grid_g2p_t<double>(p, b, c, cb, st);
+  return grid_g2p_t<float>(p, b, c, cb, st);
+}
+
+int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+                const fsi_coupling_buffers* cb, void* stream) {
+  if (int rc = check_mpm(p, b)) return rc;
+  FSI_REQUIRE(c && cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters && b->x &&
+                  b->v && b->active,
+              "contact: null buffer");
+  hipStream_t st = (hipStream_t)stream;
+  if (p->real == FSI_REAL_F64) return contact_t<double>(p, b, c, cb, st);
+  return contact_t<float>(p, b, c, cb, st);
+}
+
+int fsi_polar_rotation(int32_t real, int32_t n, const void* F, void* R, void* stream) {
+  FSI_REQUIRE(F && R && n >= 0, "polar_rotation: bad arguments");
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (real == FSI_REAL_F64)
+    hipLaunchKernelGGL(k_polar<double>, dim3(blocks_for(n)), dim3(kBlock), 0, st, (int)n,
+                       (const double*)F, (double*)R);
+  else
+    hipLaunchKernelGGL(k_polar<float>, dim3(blocks_for(n)), dim3(kBlock), 0, st, (int)n,
+                       (const float*)F, (float*)R);
+  FSI_TRY_LAUNCH("polar_rotation");
+  return 0;
+}
+
+}  // extern "C"

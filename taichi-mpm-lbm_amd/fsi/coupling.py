@@ -1,0 +1,282 @@
+"""Explicit partitioned penalty coupling on the GPU (reference: fsi/coupling.py).
+
+Same constructor, methods, accessors and exceptions as the reference's
+``LBMMpmCoupler`` (fsi/coupling.py:11-496). The coupling kernels are fused
+into the MPM passes: velocity interpolation, penalty force, reaction and
+occupancy spreading run inside P2G, contact damping inside G2P, and the
+occupancy clamp, lattice force reset, coupling-force application and
+immersed-boundary drag are one cell pass before the lattice step.
+
+Per coupled step (fsi/coupling.py:135-173):
+  clear coupling fields
+  S x [clear grid; P2G + coupling; grid update + G2P (+ contact)]
+  force assembly (clamp phi, reset force, add coupling force, IB drag)
+  lattice step
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .config import CouplingConfig
+from .lbm3d import LBMSolver3D
+from .mpm3d import MPMSolver3D
+
+# counter slots of the device counter array
+_UNSUPPORTED, _PARTIAL, _CLIPPED, _CONTACT, _DAMPED, _IB_ACTIVE, _IB_CLIPPED = range(7)
+
+
+class LBMMpmCoupler:
+    """Explicit partitioned penalty coupling between the GPU LBM and MPM solvers."""
+
+    def __init__(self, config: CouplingConfig, lbm: LBMSolver3D, mpm: MPMSolver3D):
+        self.config = config
+        self.config.validate()
+        self.lbm = lbm
+        self.mpm = mpm
+        self._validate_solver_contract()
+        self.enabled = bool(config.enabled)
+        self.build_solid_volume_fraction = bool(config.build_solid_volume_fraction)
+        self.immersed_boundary_enabled = bool(config.immersed_boundary_enabled)
+        self.contact_enabled = bool(config.contact_enabled)
+        self.substeps = int(config.mpm_substeps_per_lbm_step)
+
+        self.nx = int(lbm.nx)
+        self.ny = int(lbm.ny)
+        self.nz = int(lbm.nz)
+        self.inv_dx = 1.0 / float(mpm.dx)
+        self.cell_volume = float(mpm.dx) ** 3
+
+        if lbm.real_dtype != mpm.real_dtype:
+            raise ValueError("LBM and MPM solvers must use the same real type for coupling.")
+        n = self.nx * self.ny * self.nz
+        dev, rd = lbm.device, lbm.real_dtype
+        self._np_real = np.float64 if rd == torch.float64 else np.float32
+        # (force density xyz, phi) per cell
+        self.cforce = torch.zeros(4 * n, dtype=rd, device=dev)
+        self.ib_force = torch.zeros(3 * n, dtype=rd, device=dev)
+        self.ib_total = torch.zeros(3, dtype=rd, device=dev)
+        cap = mpm.max_particles
+        self.valid_weight = torch.zeros(cap, dtype=rd, device=dev)
+        self.coupling_mask = torch.zeros(cap, dtype=torch.int32, device=dev)
+        self.contact_mask = torch.zeros(cap, dtype=torch.int32, device=dev)
+        self.counters = torch.zeros(8, dtype=torch.int32, device=dev)
+        self.coupled_step = 0
+        self._cbuf = nat.CouplingBuffers(
+            nat.ptr(lbm.solid), nat.ptr(lbm.vel), nat.ptr(self.cforce), nat.ptr(self.valid_weight),
+            nat.ptr(self.coupling_mask), nat.ptr(self.contact_mask), nat.ptr(self.counters),
+        )
+
+    def _validate_solver_contract(self) -> None:
+        # fsi/coupling.py:42-56
+        if self.config.kernel != "quadratic":
+            raise ValueError(f"Unsupported coupling kernel: {self.config.kernel}.")
+        if (self.lbm.nx, self.lbm.ny, self.lbm.nz) != (self.mpm.nx, self.mpm.ny, self.mpm.nz):
+            raise ValueError(
+                "LBM and MPM grids must match for Step 5 coupling. "
+                f"Got LBM={(self.lbm.nx, self.lbm.ny, self.lbm.nz)}, "
+                f"MPM={(self.mpm.nx, self.mpm.ny, self.mpm.nz)}."
+            )
+        if self.mpm.dx <= 0.0:
+            raise ValueError("MPM dx must be positive.")
+
+    def _validate_initialized(self) -> None:
+        if not getattr(self.lbm, "_initialized", False):
+            raise RuntimeError("LBMSolver3D.initialize must be called before coupling.")
+        if not getattr(self.mpm, "_initialized", False):
+            raise RuntimeError("MPMSolver3D particle initialization must be called before coupling.")
+
+    # ------------------------------------------------------- parameters --
+
+    def effective_gamma(self) -> float:
+        # fsi/coupling.py:247-253 (gamma is a float32 field in the reference)
+        gamma = float(np.float32(self.config.gamma))
+        ramp_steps = int(self.config.gamma_ramp_steps)
+        if ramp_steps <= 0:
+            return gamma
+        return gamma * min(1.0, float(self.coupled_step + 1) / float(ramp_steps))
+
+    def _kernel_gamma(self) -> float:
+        # fsi/coupling.py:321-330: the ramp evaluated in the kernel's float type
+        rd = self._np_real
+        gamma = rd(self.config.gamma)
+        ramp_steps = int(self.config.gamma_ramp_steps)
+        if ramp_steps > 0:
+            ramp = (rd(self.coupled_step) + rd(1.0)) / rd(ramp_steps)
+            gamma = gamma * min(rd(1.0), ramp)
+        return float(gamma)
+
+    def _cparams(self, dt_ratio: float, build: bool, enabled: bool) -> nat.CouplingParams:
+        c = self.config
+        p = nat.CouplingParams()
+        p.enabled = 1 if enabled else 0
+        p.build_fraction = 1 if build else 0
+        p.gamma_eff = self._kernel_gamma()
+        p.dt_ratio = float(self._np_real(dt_ratio))
+        p.cell_volume = float(self._np_real(self.cell_volume))
+        p.min_valid_weight = float(c.min_valid_weight)
+        p.use_relative_velocity_limit = 0 if c.relative_velocity_limit is None else 1
+        p.relative_velocity_limit = 0.0 if c.relative_velocity_limit is None else float(c.relative_velocity_limit)
+        p.use_force_limit = 0 if c.force_limit is None else 1
+        p.force_limit = 0.0 if c.force_limit is None else float(c.force_limit)
+        p.contact_enabled = 1 if self.contact_enabled else 0
+        p.contact_threshold = float(c.contact_fraction_threshold)
+        p.contact_damping = float(c.contact_velocity_damping)
+        return p
+
+    def _assembly(self, coupling: bool, clamp: bool) -> nat.ForceAssembly:
+        c = self.config
+        a = nat.ForceAssembly()
+        a.coupling = 1 if coupling else 0
+        a.clamp_fraction = 1 if clamp else 0
+        a.ib_enabled = 1 if self.immersed_boundary_enabled else 0
+        a.ib_drag = float(c.immersed_boundary_drag)
+        a.ib_threshold = float(c.immersed_boundary_fraction_threshold)
+        a.ib_max_force = 0.0 if c.immersed_boundary_max_force is None else float(c.immersed_boundary_max_force)
+        a.ib_use_max = 0 if c.immersed_boundary_max_force is None else 1
+        a.base_force = nat.c_double3(*[float(x) for x in self.lbm.config.force])
+        return a
+
+    # ------------------------------------------------------------ steps --
+
+    def step(self, lbm_dt: float) -> None:
+        """Run one explicit coupled LBM step with the configured MPM substeps."""
+        lbm_dt_value = float(lbm_dt)
+        if lbm_dt_value <= 0.0:
+            raise ValueError("lbm_dt must be positive.")
+        self._validate_initialized()
+
+        self.lbm._ensure_macro()  # the substeps interpolate the lattice velocity of this state
+        self.clear_coupling_fields()
+        mpm_dt = lbm_dt_value / float(self.substeps)
+        dt_ratio = mpm_dt / lbm_dt_value
+        mp = self.mpm.params(self.mpm._dt_value(mpm_dt))
+        mb = self.mpm._buffers
+        for substep in range(self.substeps):
+            build = substep == 0 and self.build_solid_volume_fraction
+            cp = self._cparams(dt_ratio, build, self.enabled)
+            nat.call("fsi_mpm_clear_grid", nat.ref(mp), nat.ref(mb))
+            nat.call("fsi_mpm_p2g", nat.ref(mp), nat.ref(mb), nat.ref(cp), nat.ref(self._cbuf), 1, 1)
+            nat.call("fsi_mpm_grid_g2p", nat.ref(mp), nat.ref(mb), nat.ref(cp), nat.ref(self._cbuf))
+
+        # clamp phi, reset the lattice force, add the coupling force and IB drag
+        self.lbm._before_force_update()
+        a = self._assembly(self.enabled, self.enabled and self.build_solid_volume_fraction)
+        nat.call("fsi_force_assemble", nat.ref(self.lbm._params), nat.ref(self.lbm._buffers),
+                 nat.ref(a), nat.ptr(self.cforce), nat.ptr(self.ib_force), nat.ptr(self.ib_total),
+                 nat.ptr(self.counters), 1)
+        uniform = None
+        if not self.enabled and not self.immersed_boundary_enabled:
+            uniform = self.lbm._rounded(self.lbm.config.force)
+        self.lbm._force_uniform = uniform
+        self.lbm.step()
+        if self.enabled:
+            self.coupled_step += 1
+
+    def compute_coupling_forces(self, dt_ratio: float = 1.0) -> None:
+        """Compute particle force and fluid reaction force without stepping (coupling.py:175-196)."""
+        dt_ratio_value = float(dt_ratio)
+        if dt_ratio_value < 0.0:
+            raise ValueError("dt_ratio must be non-negative.")
+        self._validate_initialized()
+        self.lbm._ensure_macro()
+        self.clear_coupling_fields()
+        self.mpm.clear_particle_force()
+        if not self.enabled:
+            return
+        mp = self.mpm.params(self.mpm.dt_default)
+        mb = self.mpm._buffers
+        cp = self._cparams(dt_ratio_value, self.build_solid_volume_fraction, True)
+        nat.call("fsi_mpm_p2g", nat.ref(mp), nat.ref(mb), nat.ref(cp), nat.ref(self._cbuf), 1, 0)
+        a = self._assembly(False, self.build_solid_volume_fraction)
+        nat.call("fsi_force_assemble", nat.ref(self.lbm._params), nat.ref(self.lbm._buffers),
+                 nat.ref(a), nat.ptr(self.cforce), nat.ptr(self.ib_force), nat.ptr(self.ib_total),
+                 nat.ptr(self.counters), 0)
+        if self.contact_enabled:
+            nat.call("fsi_contact", nat.ref(mp), nat.ref(mb), nat.ref(cp), nat.ref(self._cbuf))
+
+    def clear_coupling_fields(self) -> None:
+        # fsi/coupling.py:281-289 (+ 291-301)
+        nat.fill_zero(self.cforce)
+        nat.fill_zero(self.ib_force)
+        nat.fill_zero(self.ib_total)
+        self.clear_particle_diagnostics()
+
+    def clear_particle_diagnostics(self) -> None:
+        # fsi/coupling.py:291-301 (entries beyond the particle count are never read)
+        nat.fill_zero(self.counters)
+        nat.fill_zero(self.valid_weight)
+        nat.fill_zero(self.coupling_mask)
+        nat.fill_zero(self.contact_mask)
+
+    # -------------------------------------------------------- accessors --
+
+    def _cells(self, t: torch.Tensor, ncomp: int) -> np.ndarray:
+        return t.cpu().numpy().reshape(self.nx, self.ny, self.nz, ncomp)
+
+    def coupling_force_numpy(self) -> np.ndarray:
+        return np.ascontiguousarray(self._cells(self.cforce, 4)[..., :3])
+
+    def solid_volume_fraction_numpy(self) -> np.ndarray:
+        return np.ascontiguousarray(self._cells(self.cforce, 4)[..., 3])
+
+    def immersed_boundary_force_numpy(self) -> np.ndarray:
+        out = torch.empty_like(self.ib_force)
+        nat.call("fsi_soa_to_aos", self.lbm.real_code, self.nx * self.ny * self.nz, 3,
+                 nat.ptr(self.ib_force), nat.ptr(out))
+        return self._cells(out, 3)
+
+    def particle_valid_weights_numpy(self) -> np.ndarray:
+        return self.valid_weight[: self.mpm.particle_count()].cpu().numpy().copy()
+
+    def particle_coupling_mask_numpy(self) -> np.ndarray:
+        return self.coupling_mask[: self.mpm.particle_count()].cpu().numpy().copy()
+
+    def particle_contact_mask_numpy(self) -> np.ndarray:
+        return self.contact_mask[: self.mpm.particle_count()].cpu().numpy().copy()
+
+    def total_particle_coupling_force(self) -> np.ndarray:
+        return self.mpm.particle_forces_numpy().sum(axis=0)
+
+    def total_fluid_coupling_force(self) -> np.ndarray:
+        force_np = self.coupling_force_numpy()
+        solid_np = self.lbm.solid_numpy()
+        return force_np[solid_np == 0].sum(axis=0)
+
+    def _counters(self) -> np.ndarray:
+        return self.counters.cpu().numpy()
+
+    def immersed_boundary_diagnostics(self) -> dict[str, int | float]:
+        ctr = self._counters()
+        ib_total_force = self.ib_total.cpu().numpy().astype(np.float64)
+        return {
+            "ib_active_cell_count": int(ctr[_IB_ACTIVE]),
+            "ib_clipped_cell_count": int(ctr[_IB_CLIPPED]),
+            "ib_total_force_x": float(ib_total_force[0]),
+            "ib_total_force_y": float(ib_total_force[1]),
+            "ib_total_force_z": float(ib_total_force[2]),
+            "ib_total_force_norm": float(np.linalg.norm(ib_total_force)),
+            "contact_candidate_count": int(ctr[_CONTACT]),
+            "contact_damped_particle_count": int(ctr[_DAMPED]),
+        }
+
+    def coupling_diagnostics(self) -> dict[str, int | float]:
+        valid_weights = self.particle_valid_weights_numpy()
+        if len(valid_weights) == 0:
+            min_weight = 0.0
+            mean_weight = 0.0
+        else:
+            min_weight = float(valid_weights.min())
+            mean_weight = float(valid_weights.mean())
+        ctr = self._counters()
+        return {
+            "unsupported_particle_count": int(ctr[_UNSUPPORTED]),
+            "partial_support_particle_count": int(ctr[_PARTIAL]),
+            "clipped_particle_count": int(ctr[_CLIPPED]),
+            "min_particle_valid_weight": min_weight,
+            "mean_particle_valid_weight": mean_weight,
+            "effective_gamma": self.effective_gamma(),
+            **self.immersed_boundary_diagnostics(),
+        }

@@ -1,0 +1,329 @@
+"""D3Q19 MRT lattice Boltzmann solver on the GPU (reference: fsi/lbm3d.py).
+
+Same constructor, methods, accessor shapes/dtypes and exceptions as the
+reference's ``LBMSolver3D`` (fsi/lbm3d.py:57-534). State lives in torch device
+tensors; every update is a HIP kernel of ``libfsi_hip.so``.
+
+State and its relation to the reference's fields:
+
+* ``pop[cur]``  - populations after streaming and boundaries, the reference's
+  ``f`` (= ``F`` after ``update_macro``), SoA (19, N). FP16 storage keeps
+  (f - w rho0) * pop_scale.
+* ``rho``/``vel`` - the reference's macroscopic fields. They are materialised
+  lazily: a step does not write them; ``_ensure_macro`` runs the macro kernel
+  when an accessor, the coupler or a force update needs them.
+* ``force``     - the per-cell force field (SoA (3, N)). When the host knows
+  it is uniform on fluid cells (``_force_uniform``), the step kernel takes the
+  value as a constant and reads no force memory.
+
+The step kernel needs the previous step's (rho, u). It recomputes them from
+the populations with the current force, which is exact while the force has
+not changed since that step; the first force change after a step therefore
+materialises (rho, u) first, and the next step reads them (``macro_in``).
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .config import BoundaryConfig, LBMConfig
+
+_Q = 19
+_SUPPORTED_X_BOUNDARIES = ("periodic", "pressure", "velocity")
+# FP16 storage scale: deviations up to 2.0 fit, the subnormal range starts
+# below 2e-9
+POP_SCALE_FP16 = 2.0 ** 15
+
+
+def _boundary_mode(boundary: BoundaryConfig) -> int:
+    # fsi/lbm3d.py:16-27
+    if boundary.boundary_type == "wall":
+        raise NotImplementedError(
+            "wall face boundaries are reserved for a later step; use solid masks for now."
+        )
+    if boundary.boundary_type not in nat.BC_MODES:
+        raise ValueError(f"Unsupported boundary type: {boundary.boundary_type}.")
+    return nat.BC_MODES[boundary.boundary_type]
+
+
+def relaxation_rates(viscosity: float) -> tuple[float, float]:
+    """fsi/lbm3d.py:141-143: tau = 3 nu + 0.5, s_v = 1/tau, s_o = 8(2 - s_v)/(8 - s_v)."""
+    tau = 3.0 * viscosity + 0.5
+    s_v = 1.0 / tau
+    s_o = 8.0 * (2.0 - s_v) / (8.0 - s_v)
+    return s_v, s_o
+
+
+class LBMSolver3D:
+    """Dense single-phase D3Q19/MRT LBM solver running on one GPU."""
+
+    def __init__(self, config: LBMConfig):
+        self.config = config
+        self.config.validate()
+        self._validate_supported_config()
+
+        self.nx = int(config.nx)
+        self.ny = int(config.ny)
+        self.nz = int(config.nz)
+        self.q = _Q
+        self.n_cells = self.nx * self.ny * self.nz
+        self.bc_x_left_mode = _boundary_mode(config.x_left)
+        self.bc_x_right_mode = _boundary_mode(config.x_right)
+        self.storage = config.storage
+        self.pop_scale = POP_SCALE_FP16 if self.storage == "fp16" else 1.0
+
+        self.device = nat.device()
+        self.real_dtype = torch.float64 if self.storage == "fp64" else torch.float32
+        self.real_code = nat.REAL_F64 if self.storage == "fp64" else nat.REAL_F32
+        self._np_real = np.float64 if self.storage == "fp64" else np.float32
+        pop_dtype = {"fp32": torch.float32, "fp16": torch.float16, "fp64": torch.float64}[self.storage]
+
+        n = self.n_cells
+        dev, rd = self.device, self.real_dtype
+        self.pop = [torch.empty(_Q * n, dtype=pop_dtype, device=dev) for _ in range(2)]
+        self._cur = 0
+        self.rho = torch.empty(n, dtype=rd, device=dev)
+        self.vel = torch.empty(3 * n, dtype=rd, device=dev)
+        self.force = torch.zeros(3 * n, dtype=rd, device=dev)
+        self.solid = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.nbmask = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.face_vel = torch.zeros(6 * self.ny * self.nz, dtype=rd, device=dev)
+
+        self._params = self._make_params()
+        self._buffers = nat.LbmBuffers(
+            nat.ptr(self.solid), nat.ptr(self.nbmask), nat.ptr(self.rho), nat.ptr(self.vel),
+            nat.ptr(self.force), nat.ptr(self.face_vel),
+        )
+        self._macro_valid = False
+        self._force_dirty = False
+        self._force_uniform: tuple[float, float, float] | None = None
+        self._initialized = False
+
+    # ------------------------------------------------------------ setup --
+
+    def _validate_supported_config(self) -> None:
+        # fsi/lbm3d.py:85-108
+        if self.config.use_sparse:
+            raise NotImplementedError("Sparse LBM storage is not implemented in Step 2.")
+        for name, boundary in (("x_left", self.config.x_left), ("x_right", self.config.x_right)):
+            if boundary.boundary_type not in _SUPPORTED_X_BOUNDARIES:
+                raise NotImplementedError(
+                    f"{name}={boundary.boundary_type!r} is not implemented in Step 2."
+                )
+        for name, boundary in (
+            ("y_left", self.config.y_left),
+            ("y_right", self.config.y_right),
+            ("z_left", self.config.z_left),
+            ("z_right", self.config.z_right),
+        ):
+            if boundary.boundary_type != "periodic":
+                raise NotImplementedError(
+                    f"{name}={boundary.boundary_type!r} is not implemented in Step 2; "
+                    "use periodic Y/Z boundaries and solid masks for now."
+                )
+
+    def _make_params(self) -> nat.LbmParams:
+        c = self.config
+        p = nat.LbmParams()
+        p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
+        p.storage = nat.STORAGE_CODES[self.storage]
+        p.rho0 = float(c.rho0)
+        p.initial_velocity = nat.c_double3(*[float(v) for v in c.initial_velocity])
+        p.s_v, p.s_o = relaxation_rates(float(c.viscosity))
+        p.left_mode = self.bc_x_left_mode
+        p.right_mode = self.bc_x_right_mode
+        p.bc_rho = (nat.ctypes.c_double * 2)(float(c.x_left.rho), float(c.x_right.rho))
+        p.bc_vel[0] = nat.c_double3(*[float(v) for v in c.x_left.velocity])
+        p.bc_vel[1] = nat.c_double3(*[float(v) for v in c.x_right.velocity])
+        p.pop_scale = float(self.pop_scale)
+        return p
+
+    def initialize(self, solid_np: np.ndarray | None = None) -> None:
+        # fsi/lbm3d.py:193-203
+        if solid_np is None:
+            solid = np.zeros((self.nx, self.ny, self.nz), dtype=np.int32)
+        else:
+            solid = self._normalize_solid_mask(solid_np)
+        self.solid.copy_(torch.from_numpy(solid.reshape(-1)))
+        nat.call("fsi_lbm_init", nat.ref(self._params), nat.ref(self._buffers),
+                 nat.ptr(self.pop[0]), nat.ptr(self.pop[1]))
+        self._cur = 0
+        self._macro_valid = True
+        self._force_dirty = False
+        self._initialized = True
+        self.reset_force()
+
+    def _normalize_solid_mask(self, solid_np: np.ndarray) -> np.ndarray:
+        arr = np.asarray(solid_np)
+        expected_shape = (self.nx, self.ny, self.nz)
+        if arr.shape != expected_shape:
+            raise ValueError(f"solid_np shape must be {expected_shape}, got {arr.shape}.")
+        return np.ascontiguousarray(np.where(arr > 0, 1, 0).astype(np.int32))
+
+    @classmethod
+    def load_solid_from_txt(cls, filename: str | Path, nx: int, ny: int, nz: int) -> np.ndarray:
+        # fsi/lbm3d.py:213-217: Fortran-order text mask
+        data = np.loadtxt(filename)
+        data = np.reshape(data, (nx, ny, nz), order="F")
+        return np.ascontiguousarray(np.where(data > 0, 1, 0).astype(np.int32))
+
+    # ------------------------------------------------------------- step --
+
+    @property
+    def populations(self) -> torch.Tensor:
+        """Current population buffer (SoA, storage dtype)."""
+        return self.pop[self._cur]
+
+    def _force_arg(self) -> nat.LbmForce:
+        f = nat.LbmForce()
+        if self._force_uniform is not None:
+            f.field = None
+            f.uniform = nat.c_double3(*self._force_uniform)
+        else:
+            f.field = nat.ptr(self.force).value
+            f.uniform = nat.c_double3(0.0, 0.0, 0.0)
+        return f
+
+    def step(self) -> None:
+        # fsi/lbm3d.py:427-433
+        if not self._initialized:
+            raise RuntimeError("LBMSolver3D.initialize must be called before step().")
+        f = self._force_arg()
+        nat.call("fsi_lbm_step", nat.ref(self._params), nat.ref(self._buffers),
+                 nat.ptr(self.pop[self._cur]), nat.ptr(self.pop[1 - self._cur]), nat.ref(f),
+                 1 if self._macro_valid else 0)
+        self._cur = 1 - self._cur
+        self._macro_valid = False
+        self._force_dirty = False
+
+    def _ensure_macro(self) -> None:
+        """Materialise rho/vel of the current populations (update_macro, lbm3d.py:409-425)."""
+        if self._macro_valid:
+            return
+        if self._force_dirty:  # cannot happen: force updates materialise first
+            raise RuntimeError("internal: macroscopic state lost after a force update")
+        f = self._force_arg()
+        nat.call("fsi_lbm_macro", nat.ref(self._params), nat.ref(self._buffers),
+                 nat.ptr(self.pop[self._cur]), nat.ref(f))
+        self._macro_valid = True
+
+    def _before_force_update(self) -> None:
+        if self._initialized:
+            self._ensure_macro()
+        self._force_dirty = True
+
+    # -------------------------------------------------------- force API --
+
+    def _force_op(self, op: int, value=(0.0, 0.0, 0.0)) -> None:
+        self._before_force_update()
+        v = (nat.ctypes.c_double * 3)(*[float(x) for x in value])
+        nat.call("fsi_lbm_force_op", nat.ref(self._params), nat.ref(self._buffers), op, v,
+                 nat.ptr(self.force), nat.ptr(self.force))
+
+    def _rounded(self, value) -> tuple[float, float, float]:
+        arr = np.asarray(value, dtype=self._np_real)
+        return float(arr[0]), float(arr[1]), float(arr[2])
+
+    def clear_force(self) -> None:
+        self._force_op(0)
+        self._force_uniform = (0.0, 0.0, 0.0)
+
+    def reset_force(self) -> None:
+        # fsi/lbm3d.py:505-510: fluid = LBMConfig.force, solid = 0
+        value = self._rounded(self.config.force)
+        self._force_op(1, value)
+        self._force_uniform = value
+
+    def set_uniform_force(self, force: tuple[float, float, float]) -> None:
+        value = self._rounded(self._validate_force_vector(force, "force"))
+        self._force_op(1, value)
+        self._force_uniform = value
+
+    def add_uniform_force(self, force: tuple[float, float, float]) -> None:
+        value = self._rounded(self._validate_force_vector(force, "force"))
+        self._force_op(2, value)
+        if self._force_uniform is not None:
+            old = np.asarray(self._force_uniform, dtype=self._np_real)
+            self._force_uniform = tuple(float(x) for x in old + np.asarray(value, self._np_real))
+
+    def set_force_from_numpy(self, force_np: np.ndarray) -> None:
+        force = self._normalize_force_field(force_np)
+        self._before_force_update()
+        staged = torch.from_numpy(force.reshape(-1)).to(self.device)
+        nat.call("fsi_lbm_force_import", nat.ref(self._params), nat.ref(self._buffers),
+                 nat.ptr(staged))
+        self._force_uniform = None
+
+    def zero_force_on_solid(self) -> None:
+        self._force_op(3)
+
+    def set_force_field(self, force: torch.Tensor, uniform: tuple | None = None) -> None:
+        """Device-side replacement of the whole force field (used by the coupler)."""
+        self._before_force_update()
+        if force.data_ptr() != self.force.data_ptr():
+            self.force.copy_(force)
+        self._force_uniform = uniform
+
+    def total_force(self) -> np.ndarray:
+        force_np = self.force_numpy()
+        solid_np = self.solid_numpy()
+        return force_np[solid_np == 0].sum(axis=0)
+
+    @staticmethod
+    def _validate_force_vector(force, name: str) -> tuple[float, float, float]:
+        arr = np.asarray(force, dtype=np.float32)
+        if arr.shape != (3,):
+            raise ValueError(f"{name} must have 3 components.")
+        return float(arr[0]), float(arr[1]), float(arr[2])
+
+    def _normalize_force_field(self, force_np: np.ndarray) -> np.ndarray:
+        arr = np.asarray(force_np, dtype=np.float32)
+        expected_shape = (self.nx, self.ny, self.nz, 3)
+        if arr.shape != expected_shape:
+            raise ValueError(f"force_np shape must be {expected_shape}, got {arr.shape}.")
+        return np.ascontiguousarray(arr)
+
+    # -------------------------------------------------------- accessors --
+
+    def _aos(self, soa: torch.Tensor, ncomp: int) -> np.ndarray:
+        out = torch.empty(soa.numel(), dtype=soa.dtype, device=self.device)
+        nat.call("fsi_soa_to_aos", self.real_code, self.n_cells, ncomp, nat.ptr(soa), nat.ptr(out))
+        return out.cpu().numpy().reshape(self.nx, self.ny, self.nz, ncomp)
+
+    def density_numpy(self) -> np.ndarray:
+        self._ensure_macro()
+        return self.rho.cpu().numpy().reshape(self.nx, self.ny, self.nz)
+
+    def velocity_numpy(self) -> np.ndarray:
+        self._ensure_macro()
+        return self._aos(self.vel, 3)
+
+    def solid_numpy(self) -> np.ndarray:
+        return self.solid.cpu().numpy().reshape(self.nx, self.ny, self.nz)
+
+    def distribution_numpy(self) -> np.ndarray:
+        out = torch.empty(self.n_cells * _Q, dtype=self.real_dtype, device=self.device)
+        nat.call("fsi_lbm_export_populations", nat.ref(self._params), nat.ptr(self.pop[self._cur]),
+                 nat.ptr(out))
+        return out.cpu().numpy().reshape(self.nx, self.ny, self.nz, _Q)
+
+    def force_numpy(self) -> np.ndarray:
+        return self._aos(self.force, 3)
+
+    def total_mass(self) -> float:
+        rho_np = self.density_numpy()
+        solid_np = self.solid_numpy()
+        return float(rho_np[solid_np == 0].sum())
+
+    def max_velocity_norm(self) -> float:
+        velocity_np = self.velocity_numpy()
+        return float(np.linalg.norm(velocity_np, axis=-1).max())
+
+    # device views used by the coupler
+    def macro_tensors(self) -> tuple[torch.Tensor, torch.Tensor]:
+        self._ensure_macro()
+        return self.rho, self.vel

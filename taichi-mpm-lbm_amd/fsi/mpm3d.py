@@ -1,0 +1,275 @@
+"""MLS-MPM elastic solid on the GPU (reference: fsi/mpm3d.py).
+
+Same constructor, methods, accessor shapes/dtypes and exceptions as the
+reference's ``MPMSolver3D`` (fsi/mpm3d.py:9-368). Particle state is SoA in
+device tensors of ``max_particles`` entries per component; the background
+grid is one AoS (momentum/velocity xyz, mass) tensor per node. A substep is
+three launches: grid clear, P2G, grid update + G2P.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .config import MPMConfig
+
+
+class MPMSolver3D:
+    """Standalone dense 3D elastic MLS-MPM solver running on one GPU.
+
+    Coordinates are grid/lattice coordinates, as in the reference.
+    """
+
+    def __init__(self, config: MPMConfig, precision: str = "fp32"):
+        self.config = config
+        self.config.validate()
+        if precision not in ("fp32", "fp64"):
+            raise ValueError(f"Unsupported MPM precision: {precision}.")
+
+        self.nx = int(config.nx)
+        self.ny = int(config.ny)
+        self.nz = int(config.nz)
+        self.max_particles = int(config.max_particles)
+        self.dx = float(config.dx)
+        self.inv_dx = 1.0 / self.dx
+        self.dt_default = float(config.dt)
+        self.boundary_width = int(config.boundary_width)
+        self.boundary_damping = float(config.boundary_damping)
+        self.apic = bool(config.apic)
+        # fsi/mpm3d.py:33-38
+        self.mu = config.youngs_modulus / (2.0 * (1.0 + config.poisson_ratio))
+        self.lam = (
+            config.youngs_modulus
+            * config.poisson_ratio
+            / ((1.0 + config.poisson_ratio) * (1.0 - 2.0 * config.poisson_ratio))
+        )
+
+        self.precision = precision
+        self.device = nat.device()
+        self.real_dtype = torch.float64 if precision == "fp64" else torch.float32
+        self.real_code = nat.REAL_F64 if precision == "fp64" else nat.REAL_F32
+        self._np_real = np.float64 if precision == "fp64" else np.float32
+        cap, dev, rd = self.max_particles, self.device, self.real_dtype
+        self.x = torch.zeros(3 * cap, dtype=rd, device=dev)
+        self.v = torch.zeros(3 * cap, dtype=rd, device=dev)
+        self.C = torch.zeros(9 * cap, dtype=rd, device=dev)
+        self.F = torch.zeros(9 * cap, dtype=rd, device=dev)
+        self.mass = torch.zeros(cap, dtype=rd, device=dev)
+        self.volume = torch.zeros(cap, dtype=rd, device=dev)
+        self.active = torch.zeros(cap, dtype=torch.int32, device=dev)
+        self.particle_force = torch.zeros(3 * cap, dtype=rd, device=dev)
+        self.grid = torch.zeros(4 * self.nx * self.ny * self.nz, dtype=rd, device=dev)
+        self.num_particles = 0
+        self._buffers = nat.MpmBuffers(
+            nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
+            nat.ptr(self.mass), nat.ptr(self.volume), nat.ptr(self.active),
+            nat.ptr(self.particle_force), nat.ptr(self.grid),
+        )
+        self._initialized = False
+
+    # ----------------------------------------------------------- set-up --
+
+    def initialize_particles_box(
+        self,
+        lower: tuple[float, float, float],
+        upper: tuple[float, float, float],
+        spacing: float | tuple[float, float, float] | None = None,
+        initial_velocity: tuple[float, float, float] = (0.0, 0.0, 0.0),
+    ) -> int:
+        # fsi/mpm3d.py:69-96
+        lower_np = self._validate_vec3(lower, "lower")
+        upper_np = self._validate_vec3(upper, "upper")
+        if np.any(upper_np <= lower_np):
+            raise ValueError("upper must be greater than lower in every dimension.")
+        spacing_np = self._default_spacing() if spacing is None else self._validate_spacing(spacing)
+        axes = [np.arange(lower_np[a], upper_np[a], spacing_np[a]) for a in range(3)]
+        if any(len(axis_values) == 0 for axis_values in axes):
+            raise ValueError("particle box produced no particles.")
+        mesh = np.meshgrid(*axes, indexing="ij")
+        positions = np.stack([component.ravel() for component in mesh], axis=1).astype(np.float32)
+        particle_volume = float(np.prod(spacing_np))
+        velocities = np.broadcast_to(
+            self._validate_vec3(initial_velocity, "initial_velocity"), positions.shape
+        ).astype(np.float32)
+        return self.initialize_particles_from_numpy(
+            positions, velocities=velocities, particle_volume=particle_volume
+        )
+
+    def initialize_particles_from_numpy(
+        self,
+        positions: np.ndarray,
+        velocities: np.ndarray | None = None,
+        particle_mass: float | None = None,
+        particle_volume: float | None = None,
+    ) -> int:
+        # fsi/mpm3d.py:98-165
+        pos = np.asarray(positions, dtype=np.float32)
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise ValueError("positions must have shape (n, 3).")
+        count = int(pos.shape[0])
+        if count <= 0:
+            raise ValueError("positions must contain at least one particle.")
+        if count > self.max_particles:
+            raise ValueError(f"particle count {count} exceeds max_particles={self.max_particles}.")
+        if velocities is None:
+            vel = np.zeros((count, 3), dtype=np.float32)
+        else:
+            vel = np.asarray(velocities, dtype=np.float32)
+            if vel.shape != (count, 3):
+                raise ValueError(f"velocities must have shape {(count, 3)}, got {vel.shape}.")
+        volume_value = (
+            float(particle_volume)
+            if particle_volume is not None
+            else self.dx ** 3 / float(self.config.particles_per_cell ** 3)
+        )
+        if volume_value <= 0.0:
+            raise ValueError("particle_volume must be positive.")
+        mass_value = (
+            float(particle_mass) if particle_mass is not None else float(self.config.density) * volume_value
+        )
+        if mass_value <= 0.0:
+            raise ValueError("particle_mass must be positive.")
+
+        cap = self.max_particles
+        rd = self._np_real
+        # host staging in SoA planes; host values pass through float32 first
+        # because the reference's fields are float32
+        x = np.zeros((3, cap), rd)
+        v = np.zeros((3, cap), rd)
+        F = np.zeros((9, cap), rd)
+        x[:, :count] = pos.T
+        v[:, :count] = vel.T
+        for d in (0, 4, 8):
+            F[d, :count] = 1.0
+        mass = np.zeros(cap, rd)
+        vol = np.zeros(cap, rd)
+        mass[:count] = np.float32(mass_value)
+        vol[:count] = np.float32(volume_value)
+        active = np.zeros(cap, np.int32)
+        active[:count] = 1
+        self.x.copy_(torch.from_numpy(x.reshape(-1)))
+        self.v.copy_(torch.from_numpy(v.reshape(-1)))
+        self.F.copy_(torch.from_numpy(F.reshape(-1)))
+        nat.fill_zero(self.C)
+        self.mass.copy_(torch.from_numpy(mass))
+        self.volume.copy_(torch.from_numpy(vol))
+        self.active.copy_(torch.from_numpy(active))
+        nat.fill_zero(self.particle_force)
+        self.num_particles = count
+        self._initialized = True
+        return count
+
+    def _default_spacing(self) -> np.ndarray:
+        return np.full(3, self.dx / float(self.config.particles_per_cell), dtype=np.float32)
+
+    @staticmethod
+    def _validate_vec3(values, name: str) -> np.ndarray:
+        arr = np.asarray(values, dtype=np.float32)
+        if arr.shape != (3,):
+            raise ValueError(f"{name} must have 3 components.")
+        return arr
+
+    @staticmethod
+    def _validate_spacing(spacing) -> np.ndarray:
+        if np.isscalar(spacing):
+            arr = np.full(3, float(spacing), dtype=np.float32)
+        else:
+            arr = np.asarray(spacing, dtype=np.float32)
+        if arr.shape != (3,):
+            raise ValueError("spacing must be a scalar or have 3 components.")
+        if np.any(arr <= 0.0):
+            raise ValueError("spacing must be positive.")
+        return arr
+
+    # ---------------------------------------------------------- stepping --
+
+    def params(self, dt: float) -> nat.MpmParams:
+        c = self.config
+        p = nat.MpmParams()
+        p.nx, p.ny, p.nz = self.nx, self.ny, self.nz
+        p.real = self.real_code
+        p.num_particles = self.num_particles
+        p.capacity = self.max_particles
+        p.dx = self.dx
+        p.inv_dx = self.inv_dx
+        p.dt = float(dt)
+        p.two_mu = 2.0 * self.mu
+        p.lam = self.lam
+        p.gravity = nat.c_double3(*[float(g) for g in c.gravity])
+        p.boundary_width = self.boundary_width
+        p.boundary_damping = self.boundary_damping
+        p.apic = 1 if self.apic else 0
+        p.clamp_lower = 0.5 * self.dx
+        p.clamp_upper = nat.c_double3((self.nx - 1.5) * self.dx, (self.ny - 1.5) * self.dx,
+                                      (self.nz - 1.5) * self.dx)
+        return p
+
+    def _dt_value(self, dt: float | None) -> float:
+        dt_value = self.dt_default if dt is None else float(dt)
+        if dt_value <= 0.0:
+            raise ValueError("dt must be positive.")
+        # the reference stores dt in a float32 field
+        return float(self._np_real(dt_value))
+
+    def substep(self, dt: float | None = None) -> None:
+        # fsi/mpm3d.py:189-203
+        if not self._initialized:
+            raise RuntimeError(
+                "MPMSolver3D.initialize_particles_box or initialize_particles_from_numpy "
+                "must be called before substep()."
+            )
+        p = self.params(self._dt_value(dt))
+        nat.call("fsi_mpm_clear_grid", nat.ref(p), nat.ref(self._buffers))
+        nat.call("fsi_mpm_p2g", nat.ref(p), nat.ref(self._buffers), None, None, 0, 1)
+        nat.call("fsi_mpm_grid_g2p", nat.ref(p), nat.ref(self._buffers), None, None)
+
+    def clear_particle_force(self) -> None:
+        # fsi/mpm3d.py:211-214
+        nat.fill_zero(self.particle_force)
+
+    # --------------------------------------------------------- accessors --
+
+    def _plane(self, t: torch.Tensor, ncomp: int) -> np.ndarray:
+        n = self.num_particles
+        return t.view(ncomp, self.max_particles)[:, :n].cpu().numpy().T.copy()
+
+    def particle_count(self) -> int:
+        return int(self.num_particles)
+
+    def positions_numpy(self) -> np.ndarray:
+        return self._plane(self.x, 3)
+
+    def velocities_numpy(self) -> np.ndarray:
+        return self._plane(self.v, 3)
+
+    def particle_forces_numpy(self) -> np.ndarray:
+        return self._plane(self.particle_force, 3)
+
+    def deformation_gradients_numpy(self) -> np.ndarray:
+        return self._plane(self.F, 9).reshape(-1, 3, 3)
+
+    def active_numpy(self) -> np.ndarray:
+        return self.active[: self.num_particles].cpu().numpy().copy()
+
+    def masses_numpy(self) -> np.ndarray:
+        return self.mass[: self.num_particles].cpu().numpy().copy()
+
+    def center_of_mass(self) -> np.ndarray:
+        # fsi/mpm3d.py:353-362
+        count = self.particle_count()
+        if count == 0:
+            return np.zeros(3, dtype=self._np_real)
+        positions = self.positions_numpy()
+        masses = self.masses_numpy()
+        total_mass = masses.sum()
+        if total_mass <= 0.0:
+            return np.zeros(3, dtype=self._np_real)
+        return (positions * masses[:, None]).sum(axis=0) / total_mass
+
+    def max_velocity_norm(self) -> float:
+        velocities = self.velocities_numpy()
+        if len(velocities) == 0:
+            return 0.0
+        return float(np.linalg.norm(velocities, axis=1).max())

@@ -1,0 +1,143 @@
+"""LBM parity: HIP kernels against the CPU oracle (fsi/lbm3d.py restated).
+
+Gates, per field (populations as deviations f - w rho0, density, velocity),
+with rel = max|a - b| / max|b|:
+
+* FP64 build vs FP64 oracle: rel <= 1e-10. The FP64 instantiation runs the
+  same kernel code as FP32, so this pins the kernel logic (collision, push
+  stream, bounce-back, x-face boundaries, force handling, macro fields).
+* FP32 build vs FP64 oracle: within twice the error the reference's own FP32
+  arithmetic (the oracle's FP32 build, in the reference's operation order)
+  makes against FP64, plus 1e-7 - i.e. as accurate as the reference.
+* FP16 shifted storage vs FP64 oracle: velocity rel <= 3e-3, density rel <= 1e-5.
+"""
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from parity import W, channel_config, lbm_fields, porous_mask, rel_inf, run_lbm
+
+from cases import periodic_lbm_config
+
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("gpu_available")]
+
+
+def _local_force(cfg):
+    f = np.zeros((cfg.nx, cfg.ny, cfg.nz, 3), np.float32)
+    f[2:6, :, :, 0] = 1.0e-6
+    f[:, 3:5, :, 1] = -2.0e-7
+    return f
+
+
+def _change_force(s):
+    s.add_uniform_force((5.0e-7, 0.0, 1.0e-7))
+
+
+def _scenarios():
+    periodic = periodic_lbm_config(force=(1.0e-6, 0.0, 0.0))
+    chan, chan_solid = channel_config()
+    porous_cfg = periodic_lbm_config(nx=16, ny=16, nz=16, viscosity=0.05 / 3, force=(1.0e-5, 0.0, 0.0))
+    porous = porous_mask((16, 16, 16), seed=3, porosity=0.8, rmin=1.5, rmax=3.0)
+    pressure_both, pb_solid = channel_config(
+        x_left=dataclasses.replace(chan.x_left, boundary_type="pressure", rho=1.001),
+        x_right=dataclasses.replace(chan.x_right, boundary_type="pressure", rho=0.999),
+    )
+    local = periodic_lbm_config(initial_velocity=(1.0e-3, 0.0, 0.0))
+    return {
+        "periodic_force": dict(cfg=periodic, steps=20),
+        "channel_velocity_pressure": dict(cfg=chan, steps=60, solid=chan_solid),
+        "porous_body_force": dict(cfg=porous_cfg, steps=40, solid=porous),
+        "pressure_pressure": dict(cfg=pressure_both, steps=40, solid=pb_solid),
+        "local_force_changed": dict(cfg=local, steps=12, force_field=_local_force(local),
+                                    force_change=_change_force),
+    }
+
+
+SCENARIOS = _scenarios()
+
+
+def _run(api_factory, sc, storage=None):
+    cfg = sc["cfg"] if storage is None else dataclasses.replace(sc["cfg"], storage=storage)
+    s = run_lbm(api_factory, cfg, sc["steps"], solid=sc.get("solid"),
+                force_field=sc.get("force_field"), force_change=sc.get("force_change"))
+    return lbm_fields(s, cfg.rho0)
+
+
+@pytest.fixture(scope="module")
+def oracle_runs():
+    out = {}
+    for name, sc in SCENARIOS.items():
+        out[name] = {p: _run(lambda c, p=p: po.OracleLBM(c, p), sc) for p in ("f32", "f64")}
+    return out
+
+
+@pytest.mark.parametrize("name", list(SCENARIOS))
+def test_lbm_fp64_build_matches_fp64_oracle(name, oracle_runs):
+    import fsi
+
+    got = _run(fsi.LBMSolver3D, SCENARIOS[name], storage="fp64")
+    ref = oracle_runs[name]["f64"]
+    for key in ("dev", "rho", "u"):
+        err = rel_inf(got[key], ref[key])
+        print(f"{name} fp64 {key}: rel {err:.3e}")
+        assert err <= 1e-10, (name, key, err)
+
+
+@pytest.mark.parametrize("name", list(SCENARIOS))
+def test_lbm_fp32_as_accurate_as_reference_arithmetic(name, oracle_runs):
+    import fsi
+
+    got = _run(fsi.LBMSolver3D, SCENARIOS[name], storage="fp32")
+    ref64 = oracle_runs[name]["f64"]
+    ref32 = oracle_runs[name]["f32"]
+    for key in ("dev", "rho", "u"):
+        err_gpu = rel_inf(got[key], ref64[key])
+        err_ref = rel_inf(ref32[key], ref64[key])
+        err_vs32 = rel_inf(got[key], ref32[key])
+        print(f"{name} fp32 {key}: gpu-vs-f64 {err_gpu:.3e} ref32-vs-f64 {err_ref:.3e} "
+              f"gpu-vs-ref32 {err_vs32:.3e}")
+        assert err_gpu <= 2.0 * err_ref + 1e-7, (name, key, err_gpu, err_ref)
+
+
+@pytest.mark.parametrize("name", list(SCENARIOS))
+def test_lbm_fp16_shifted_storage_error(name, oracle_runs):
+    import fsi
+
+    got = _run(fsi.LBMSolver3D, SCENARIOS[name], storage="fp16")
+    ref = oracle_runs[name]["f64"]
+    eu = rel_inf(got["u"], ref["u"])
+    er = rel_inf(got["rho"], ref["rho"])
+    print(f"{name} fp16: u rel {eu:.3e} rho rel {er:.3e}")
+    assert eu <= 3e-3 and er <= 1e-5, (name, eu, er)
+
+
+def test_solid_cells_keep_initial_populations_and_zero_velocity():
+    import fsi
+
+    cfg, solid = channel_config()
+    s = run_lbm(fsi.LBMSolver3D, cfg, 10, solid=solid)
+    f = s.distribution_numpy()
+    assert np.array_equal(f[solid == 1], np.broadcast_to(np.float32(W * cfg.rho0), f[solid == 1].shape))
+    u = s.velocity_numpy()
+    assert np.all(u[solid == 1] == 0.0)
+    assert np.all(s.density_numpy()[solid == 1] == np.float32(cfg.rho0))
+
+
+def test_solid_mask_and_neighbour_bits_are_exact():
+    import fsi
+
+    cfg = periodic_lbm_config(nx=12, ny=10, nz=9)
+    solid = porous_mask((12, 10, 9), seed=5, porosity=0.7, rmin=1.0, rmax=2.5)
+    s = fsi.LBMSolver3D(cfg)
+    s.initialize(solid)
+    assert np.array_equal(s.solid_numpy(), solid)
+    bits = s.nbmask.cpu().numpy().view(np.uint32).reshape(12, 10, 9)
+    expect = np.zeros_like(bits)
+    for q in range(19):
+        e = po.E[q]
+        nb = np.roll(solid, shift=(-e[0], -e[1], -e[2]), axis=(0, 1, 2))
+        expect |= (nb.astype(np.uint32) << np.uint32(q))
+    assert np.array_equal(bits, expect)

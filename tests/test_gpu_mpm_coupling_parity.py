@@ -1,0 +1,175 @@
+"""MPM and coupling parity: HIP kernels against the CPU oracle.
+
+Gates as in test_gpu_lbm_parity.py: the FP64 build matches the FP64 oracle to
+1e-9 relative per field (the grid and coupling accumulations are float atomics
+whose order differs from the oracle's particle order, so FP64 rounding noise
+is the floor); the FP32 build stays within twice the reference-order FP32
+error band against FP64 (+ a small absolute floor); integer outputs (coupling
+masks, support/clip counters, contact counts) are exact.
+"""
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from parity import coupled_scene, jittered_cube, rel_inf, sim_fields
+
+from cases import small_mpm_config
+
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("gpu_available")]
+
+
+# ------------------------------------------------------------------ MPM --
+
+
+def _spinning_cube():
+    cfg = small_mpm_config(youngs_modulus=1000.0, poisson_ratio=0.3, dt=2.0e-3,
+                           gravity=(0.0, -1.0e-3, 0.0))
+    pos = jittered_cube((5.0, 5.0, 5.0), (11.0, 11.0, 11.0), 0.5, seed=7)
+    c = pos.mean(axis=0)
+    omega = np.array([0.02, -0.05, 0.08])
+    vel = (np.cross(omega, pos - c) - 0.01 * (pos - c)).astype(np.float32)
+    return cfg, pos, vel
+
+
+def _run_mpm(solver, pos, vel, steps):
+    solver.initialize_particles_from_numpy(pos, vel, particle_mass=0.15, particle_volume=0.125)
+    for _ in range(steps):
+        solver.substep()
+    return {
+        "x": solver.positions_numpy(),
+        "v": solver.velocities_numpy(),
+        "F": solver.deformation_gradients_numpy(),
+    }
+
+
+@pytest.fixture(scope="module")
+def mpm_oracle():
+    cfg, pos, vel = _spinning_cube()
+    return {p: _run_mpm(po.OracleMPM(cfg, p), pos, vel, 60) for p in ("f32", "f64")}
+
+
+def test_mpm_fp64_matches_fp64_oracle(mpm_oracle):
+    import fsi
+
+    cfg, pos, vel = _spinning_cube()
+    got = _run_mpm(fsi.MPMSolver3D(cfg, precision="fp64"), pos, vel, 60)
+    for key in ("x", "v", "F"):
+        err = rel_inf(got[key], mpm_oracle["f64"][key])
+        print(f"mpm fp64 {key}: rel {err:.3e}")
+        assert err <= 1e-9, (key, err)
+
+
+def test_mpm_fp32_within_reference_band(mpm_oracle):
+    import fsi
+
+    cfg, pos, vel = _spinning_cube()
+    got = _run_mpm(fsi.MPMSolver3D(cfg), pos, vel, 60)
+    for key in ("x", "v", "F"):
+        e_gpu = rel_inf(got[key], mpm_oracle["f64"][key])
+        e_ref = rel_inf(mpm_oracle["f32"][key], mpm_oracle["f64"][key])
+        print(f"mpm fp32 {key}: gpu-vs-f64 {e_gpu:.3e} ref32-vs-f64 {e_ref:.3e}")
+        assert e_gpu <= 2.0 * e_ref + 2e-6, (key, e_gpu, e_ref)
+
+
+def _random_deformations(n, seed):
+    rng = np.random.default_rng(seed)
+    F = np.eye(3) + 0.3 * rng.standard_normal((n, 3, 3))
+    # near-degenerate and inverted cases exercise the FP64 recomputation
+    F[: n // 8, 2, :] = F[: n // 8, 1, :] * (1.0 + 1e-4)
+    F[n // 8: n // 4, 0, :] *= -1.0
+    F[n // 4: n // 4 + 4] = np.eye(3)
+    return F
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-5)])
+def test_svd_rotation_factor(precision, tol):
+    import torch
+
+    from fsi import _native as nat
+
+    F = _random_deformations(512, seed=11)
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    Fd = torch.tensor(F.reshape(-1), dtype=dt, device="cuda")
+    Rd = torch.empty_like(Fd)
+    nat.call("fsi_polar_rotation", nat.REAL_F64 if precision == "fp64" else nat.REAL_F32, len(F),
+             nat.ptr(Fd), nat.ptr(Rd))
+    R = Rd.cpu().numpy().astype(np.float64).reshape(-1, 3, 3)
+    for i in range(len(F)):
+        ref = po.polar_rotation(F[i].astype(np.float32 if precision == "fp32" else np.float64), "f64")
+        # proper rotation
+        assert np.allclose(R[i] @ R[i].T, np.eye(3), atol=10 * tol)
+        assert abs(np.linalg.det(R[i]) - 1.0) < 10 * tol
+        # R^T F symmetric (polar/rotation-variant SVD stretch)
+        S = R[i].T @ F[i]
+        assert np.abs(S - S.T).max() <= 50 * tol * max(1.0, np.abs(F[i]).max())
+        # same rotation as the oracle's
+        assert np.abs(R[i] - ref).max() <= 50 * tol, (i, np.abs(R[i] - ref).max())
+
+
+# ------------------------------------------------------------- coupling --
+
+
+def _run_sim(make, storage, steps, **kw):
+    cfg, solid, pos, vel = coupled_scene(storage=storage, **kw)
+    sim = make(cfg)
+    sim.initialize_lbm(solid)
+    sim.initialize_mpm_from_numpy(pos, vel, particle_mass=1.2 * 0.125, particle_volume=0.125)
+    for _ in range(steps):
+        sim.step()
+    out = sim_fields(sim)
+    out["diag"] = sim.coupler.coupling_diagnostics()
+    return out
+
+
+COUPLED = {
+    "penalty": dict(steps=6),
+    "penalty_ib_contact": dict(steps=4, ib=True, contact=True),
+}
+
+
+@pytest.fixture(scope="module")
+def coupled_oracle():
+    out = {}
+    for name, kw in COUPLED.items():
+        out[name] = {
+            p: _run_sim(lambda c, p=p: po.OracleSimulation(c, p), None, **kw) for p in ("f32", "f64")
+        }
+    return out
+
+
+FIELDS = ("dev", "rho", "u", "x", "v", "F", "pforce", "cforce", "phi", "lbm_force", "wvalid")
+
+
+@pytest.mark.parametrize("name", list(COUPLED))
+def test_coupled_fp64_matches_fp64_oracle(name, coupled_oracle):
+    import fsi
+
+    got = _run_sim(fsi.FSISimulation, "fp64", **COUPLED[name])
+    ref = coupled_oracle[name]["f64"]
+    for key in FIELDS:
+        err = rel_inf(got[key], ref[key])
+        print(f"{name} fp64 {key}: rel {err:.3e}")
+        assert err <= 1e-9, (name, key, err)
+    assert np.array_equal(got["cmask"], ref["cmask"])
+    for k in ("unsupported_particle_count", "partial_support_particle_count",
+              "clipped_particle_count", "ib_active_cell_count", "contact_candidate_count",
+              "contact_damped_particle_count"):
+        assert got["diag"][k] == ref["diag"][k], (k, got["diag"][k], ref["diag"][k])
+
+
+@pytest.mark.parametrize("name", list(COUPLED))
+def test_coupled_fp32_within_reference_band(name, coupled_oracle):
+    import fsi
+
+    got = _run_sim(fsi.FSISimulation, "fp32", **COUPLED[name])
+    ref64 = coupled_oracle[name]["f64"]
+    ref32 = coupled_oracle[name]["f32"]
+    for key in FIELDS:
+        e_gpu = rel_inf(got[key], ref64[key])
+        e_ref = rel_inf(ref32[key], ref64[key])
+        print(f"{name} fp32 {key}: gpu-vs-f64 {e_gpu:.3e} ref32-vs-f64 {e_ref:.3e}")
+        assert e_gpu <= 2.0 * e_ref + 1e-6, (name, key, e_gpu, e_ref)
+    assert np.array_equal(got["cmask"], ref32["cmask"])

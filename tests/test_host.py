@@ -1,0 +1,132 @@
+"""Host-side logic that runs without a GPU: configuration validation (the
+reference's tests/test_config.py rules), solver argument checks that raise
+before any device work, the import surface, and the C ABI symbol table."""
+
+import re
+from pathlib import Path
+
+import pytest
+
+from fsi.config import (
+    BoundaryConfig,
+    CouplingConfig,
+    LBMConfig,
+    MPMConfig,
+    OutputConfig,
+    SimulationConfig,
+)
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_defaults_match_reference():
+    # fsi/config.py:20-259 defaults
+    lbm = LBMConfig()
+    assert (lbm.nx, lbm.ny, lbm.nz, lbm.viscosity, lbm.rho0) == (64, 32, 16, 0.1, 1.0)
+    assert lbm.x_left.boundary_type == "velocity" and lbm.x_right.boundary_type == "pressure"
+    mpm = MPMConfig()
+    assert (mpm.max_particles, mpm.particles_per_cell, mpm.dx, mpm.dt) == (200_000, 2, 1.0, 0.25)
+    assert mpm.gravity == (0.0, -9.8e-4, 0.0) and mpm.boundary_width == 3
+    c = CouplingConfig()
+    assert (c.gamma, c.mpm_substeps_per_lbm_step, c.min_valid_weight) == (0.05, 4, 1.0e-6)
+    s = SimulationConfig(mpm=MPMConfig(nx=64, ny=32, nz=16))
+    s.validate()
+    assert s.mpm_dt == pytest.approx(0.25)
+
+
+@pytest.mark.parametrize(
+    "cfg",
+    [
+        BoundaryConfig("bogus"),
+        BoundaryConfig(rho=0.0),
+        LBMConfig(nx=4),
+        LBMConfig(viscosity=0.0),
+        LBMConfig(rho0=-1.0),
+        LBMConfig(force=(1.0, 2.0)),
+        LBMConfig(storage="int8"),
+        MPMConfig(max_particles=0),
+        MPMConfig(dx=0.0),
+        MPMConfig(poisson_ratio=0.5),
+        MPMConfig(material="plastic"),
+        MPMConfig(boundary_width=0),
+        MPMConfig(boundary_damping=1.5),
+        CouplingConfig(gamma=-1.0),
+        CouplingConfig(mpm_substeps_per_lbm_step=0),
+        CouplingConfig(kernel="cubic"),
+        CouplingConfig(force_limit=0.0),
+        CouplingConfig(relative_velocity_limit=-1.0),
+        CouplingConfig(gamma_ramp_steps=-1),
+        CouplingConfig(min_valid_weight=0.0),
+        CouplingConfig(min_valid_weight=1.5),
+        CouplingConfig(immersed_boundary_drag=-1.0),
+        CouplingConfig(immersed_boundary_fraction_threshold=1.1),
+        CouplingConfig(immersed_boundary_max_force=0.0),
+        CouplingConfig(contact_velocity_damping=-0.1),
+        CouplingConfig(contact_fraction_threshold=1.1),
+        OutputConfig(output_interval=0),
+        OutputConfig(output_format="hdf5"),
+        SimulationConfig(backend="tpu"),
+        SimulationConfig(num_steps=0),
+        SimulationConfig(lbm_dt=0.0),
+        SimulationConfig(mpm=MPMConfig(nx=65)),
+    ],
+)
+def test_invalid_config_raises_value_error(cfg):
+    with pytest.raises(ValueError):
+        cfg.validate()
+
+
+def test_unsupported_lattice_features_raise_before_device_work():
+    from fsi.lbm3d import LBMSolver3D
+
+    periodic = {f: BoundaryConfig("periodic") for f in ("x_left", "x_right", "y_left", "y_right", "z_left", "z_right")}
+    with pytest.raises(NotImplementedError):
+        LBMSolver3D(LBMConfig(use_sparse=True, **periodic))
+    with pytest.raises(NotImplementedError):
+        LBMSolver3D(LBMConfig(**{**periodic, "y_left": BoundaryConfig("pressure")}))
+    with pytest.raises(NotImplementedError):
+        LBMSolver3D(LBMConfig(**{**periodic, "x_left": BoundaryConfig("wall")}))
+
+
+def test_solvers_fail_loudly_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from fsi.lbm3d import LBMSolver3D
+    from fsi.mpm3d import MPMSolver3D
+
+    periodic = {f: BoundaryConfig("periodic") for f in ("x_left", "x_right", "y_left", "y_right", "z_left", "z_right")}
+    with pytest.raises(RuntimeError):
+        LBMSolver3D(LBMConfig(nx=8, ny=8, nz=8, **periodic))
+    with pytest.raises(RuntimeError):
+        MPMSolver3D(MPMConfig(nx=8, ny=8, nz=8))
+
+
+def test_import_surface():
+    import fsi
+
+    for name in ("SimulationConfig", "LBMConfig", "MPMConfig", "CouplingConfig", "BoundaryConfig",
+                 "OutputConfig"):
+        assert hasattr(fsi, name)
+    for name in ("LBMSolver3D", "MPMSolver3D", "LBMMpmCoupler", "FSISimulation",
+                 "SimulationOutputWriter"):
+        assert name in fsi.__all__
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "fsi_hip.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(fsi_\w+)\s*\(", text, re.M)))
+
+
+def test_c_abi_library_exports_every_declared_symbol():
+    """The built library loads on the CPU and exports each entry point of the header."""
+    from fsi import _native
+
+    lib = _native.load_library()
+    names = _declared_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.fsi_abi_version() == 1
+    assert set(names) >= set(_native._SIGNATURES)
