@@ -1,0 +1,260 @@
+#!/usr/bin/env python
+"""Benchmark of the coupled LBM-MPM step on one MI355X.
+
+Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
+random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
+all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
+update of the whole 256^3 lattice (one fused collide+stream launch); `value`
+is the FP32-population MLUPS (all cells), the reference's arithmetic type.
+The same scene then runs with FP16 shifted storage, and the 32^3 coupled
+penalty scene (4096 particles, 112 MPM substeps per lattice step) reports
+coupled steps/s. Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT, ROOT / "taichi-mpm-lbm_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+METRIC = "LBM MLUPS (%HBM roofline) + coupled steps/s @256³; FP16-store/learned vs FP32"
+HBM_PEAK_GBS = 8000.0
+
+
+def periodic_lbm_config(n, storage, tau=0.55, force=(1.0e-6, 0.0, 0.0)):
+    from fsi.config import BoundaryConfig, LBMConfig
+
+    faces = {f: BoundaryConfig("periodic") for f in ("x_left", "x_right", "y_left", "y_right", "z_left", "z_right")}
+    return LBMConfig(nx=n, ny=n, nz=n, viscosity=(tau - 0.5) / 3.0, rho0=1.0, force=force,
+                     storage=storage, **faces)
+
+
+def algorithmic_bytes(n_fluid, n_cells, pop_bytes):
+    # per step: read + write 19 populations per fluid cell, 1 mask byte per cell
+    return n_fluid * 2 * 19 * pop_bytes + n_cells * 1
+
+
+def time_lattice(solver, steps, warmup, launch_samples):
+    import torch
+
+    for _ in range(warmup):
+        solver.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # per-launch kernel duration: event pairs on the stream the kernel runs on
+    stream = torch.cuda.current_stream()
+    pairs = []
+    for _ in range(launch_samples):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        solver.step()
+        b.record(stream)
+        pairs.append((a, b))
+    torch.cuda.synchronize()
+    launch_ms = sum(a.elapsed_time(b) for a, b in pairs) / len(pairs)
+    return wall, launch_ms
+
+
+def read_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    path = ROOT / "profiles" / name
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(mask, budget_s=12.0, max_steps=3):
+    """The CPU oracle (C++ restatement in the reference's operation order,
+    FP32, OpenMP over the host cores) on the same 256^3 scene; bounded sample."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+
+    po.build()
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    n = mask.shape[0]
+    s = po.OracleLBM(periodic_lbm_config(n, "fp32"), "f32")
+    s.initialize(mask)
+    s.step()  # warm-up (page faults, thread start)
+    done = 0
+    t0 = time.perf_counter()
+    while done < max_steps:
+        s.step()
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(mask.size * done / dt / 1e6, 3),
+        "unit": "MLUPS",
+        "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+        "kind": "port",
+        "sample": f"{done} timed steps (+1 warm-up) of the 256^3 porous lattice in the CPU oracle, "
+                  f"FP32, reference operation order; {dt:.1f} s",
+    }
+
+
+def coupled_scene_32(storage="fp32"):
+    """32^3 coupled penalty scene: inflow u_x=0.02 (velocity face x-), pressure
+    face x+, no-slip walls as a solid shell in y/z, tau=0.6; elastic cube
+    [8,16]x[12,20]^2 of 16^3 particles (spacing 0.5, jitter U(-0.125,0.125),
+    seed 0), E=1e3, nu=0.3, density 1.2, gamma=0.5, 112 substeps."""
+    from fsi.config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
+    from fsi.scenes import channel_walls, jittered_box
+
+    n = 32
+    lbm = LBMConfig(nx=n, ny=n, nz=n, viscosity=0.1 / 3.0, rho0=1.0,
+                    x_left=BoundaryConfig("velocity", velocity=(0.02, 0.0, 0.0)),
+                    x_right=BoundaryConfig("pressure", rho=1.0), storage=storage)
+    mpm = MPMConfig(nx=n, ny=n, nz=n, max_particles=4096, dx=1.0, dt=1.0 / 112, density=1.2,
+                    youngs_modulus=1.0e3, poisson_ratio=0.3, gravity=(0.0, 0.0, 0.0),
+                    boundary_width=2)
+    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=112)
+    cfg = SimulationConfig(num_steps=100, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
+    solid = channel_walls((n, n, n))
+    pos = jittered_box((8.0, 12.0, 12.0), (16.0, 20.0, 20.0), 0.5, seed=0, jitter=0.5)
+    return cfg, solid, pos
+
+
+def time_coupled(steps, warmup):
+    import torch
+
+    import fsi
+
+    cfg, solid, pos = coupled_scene_32()
+    sim = fsi.FSISimulation(cfg)
+    sim.initialize_lbm(solid)
+    sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+    for _ in range(warmup):
+        sim.advance()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.advance()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"steps_per_s": round(steps / dt, 3), "ms_per_step": round(1e3 * dt / steps, 4),
+            "particles": int(len(pos)), "substeps": 112, "timed_steps": steps}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--launch-samples", type=int, default=200)
+    ap.add_argument("--coupled-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-coupled", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding "
+              "of the lattice is a design note, not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import fsi
+    from fsi.scenes import porous_spheres
+
+    n = args.size
+    mask, info = porous_spheres((n, n, n), porosity=0.8, rmin=4.0, rmax=10.0, seed=0)
+    n_cells = mask.size
+    n_fluid = int((mask == 0).sum())
+
+    runs = {}
+    storages = ["fp32"] if args.no_fp16 else ["fp32", "fp16"]
+    for storage in storages:
+        s = fsi.LBMSolver3D(periodic_lbm_config(n, storage))
+        s.initialize(mask)
+        wall, launch_ms = time_lattice(s, args.steps, args.warmup, args.launch_samples)
+        b = 4 if storage == "fp32" else 2
+        alg = algorithmic_bytes(n_fluid, n_cells, b)
+        runs[storage] = {
+            "mlups": n_cells * args.steps / wall / 1e6,
+            "mflups": n_fluid * args.steps / wall / 1e6,
+            "ms_per_step": 1e3 * wall / args.steps,
+            "launch_ms": launch_ms,
+            "bytes": alg,
+            "achieved_gbs": alg / (launch_ms * 1e-3) / 1e9,
+        }
+        del s
+        torch.cuda.empty_cache()
+
+    r32 = runs["fp32"]
+    traffic = read_traffic(f"lbm_step_{n}_fp32_pmc.txt")
+    line = {
+        "metric": METRIC,
+        "value": round(r32["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"D3Q19 MRT lattice {n}^3, porous sphere mask (porosity "
+                        f"{info['porosity']:.4f}, r 4-10, seed 0), periodic, g_x=1e-6, tau=0.55, "
+                        "FP32 populations",
+            "cells": n_cells,
+            "fluid_cells": n_fluid,
+            "kernel": "k_lbm_pull<float> stream+collide (fused pull-scheme MRT step)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(r32["achieved_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "mflups": round(r32["mflups"], 2),
+        "launch_ms": round(r32["launch_ms"], 5),
+    }
+    if "fp16" in runs:
+        r16 = runs["fp16"]
+        line["fp16"] = {
+            "mlups": round(r16["mlups"], 2),
+            "ms_per_step": round(r16["ms_per_step"], 5),
+            "launch_ms": round(r16["launch_ms"], 5),
+            "roofline_frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "achieved_gbs": round(r16["achieved_gbs"], 1),
+            "speedup_vs_fp32": round(r16["mlups"] / r32["mlups"], 3),
+            "traffic": read_traffic(f"lbm_step_{n}_fp16_pmc.txt"),
+        }
+    if not args.no_coupled:
+        line["coupled"] = {"workload": "32^3 coupled penalty scene, FP32, 4096 particles, 112 substeps",
+                           **time_coupled(args.coupled_steps, 2)}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(mask)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -63,8 +63,8 @@ typedef struct fsi_lbm_buffers {
   void* rho;               /* N real: macroscopic density */
   void* vel;               /* 3N real: macroscopic velocity */
   void* force;             /* 3N real: per-cell body force (fsi/lbm3d.py:126) */
-  void* face_vel;          /* 2*3*ny*nz real: velocity of the x=0 / x=nx-1 cells used by the
-                              pressure boundary of the next step */
+  void* face_vel;          /* 2 slots x 2 faces x 3 x ny*nz real: macroscopic velocity of the
+                              x=0 / x=nx-1 cells, read by the next pressure boundary */
 } fsi_lbm_buffers;
 
 /* Force seen by one step. When `field` is null, every fluid cell uses
@@ -74,25 +74,36 @@ typedef struct fsi_lbm_force {
   double uniform[3];
 } fsi_lbm_force;
 
+/* Lattice phases of fsi_lbm_step. Between steps the populations are either
+ * post-collision (after a step) or post-stream with rho/u materialised (the
+ * reference's state after update_macro; after initialize). */
+enum {
+  FSI_PHASE_STREAM_COLLIDE = 0, /* post -> post: one full step */
+  FSI_PHASE_COLLIDE = 1,        /* materialised -> post: collide the stored state */
+  FSI_PHASE_STREAM_MACRO = 2,   /* post -> materialised: stream, boundaries, rho/u */
+};
+
 /* fsi/lbm3d.py:193-203 + 219-267 (initialize: static_init, init_fields):
  * neighbour masks, rho = rho0, u = u0 (0 on solid), both population
- * buffers = feq(rho0, u). */
+ * buffers = feq(rho0, u) (the materialised state), face velocities in slot 0. */
 int fsi_lbm_init(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pop_a, void* pop_b,
                  void* stream);
 
-/* One LBM step, fsi/lbm3d.py:427-433 (collide 314-333, stream 352-361,
- * apply_boundary 363-407), fused: each fluid cell reads its populations from
- * pop_in, forms (rho, u) of the previous step (from the stored macroscopic
- * fields when macro_in != 0, else from pop_in and `force`), collides with
- * `force` and pushes to pop_out with half-way bounce-back; a second launch
- * applies the x-face boundaries on pop_out. update_macro (409-425) is not
- * run here: see fsi_lbm_macro. */
+/* The lattice update of fsi/lbm3d.py:427-433 (collide 314-333, stream
+ * 352-361 with half-way bounce-back, apply_boundary 363-407, update_macro
+ * 409-425) as one pull-scheme kernel in the given phase: pop_in -> pop_out.
+ * A full step from the post-collision state is STREAM_COLLIDE: each cell
+ * gathers its streamed populations, applies the x-face boundaries, forms
+ * rho/u with `force` (the force of the previous collision, unchanged), and
+ * collides with `force`. When the force changed since the last step the
+ * caller first runs STREAM_MACRO (writes the post-stream populations and
+ * rho/u) and then COLLIDE, which collides with the new force from the
+ * stored rho/u - exactly the reference's order. face_slot (0/1) selects the
+ * half of b->face_vel holding the previous face velocities; phases that form
+ * rho/u write the other half. */
 int fsi_lbm_step(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop_in,
-                 void* pop_out, const fsi_lbm_force* force, int32_t macro_in, void* stream);
-
-/* fsi/lbm3d.py:409-425 (update_macro): rho and u from pop and force. */
-int fsi_lbm_macro(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop,
-                  const fsi_lbm_force* force, void* stream);
+                 void* pop_out, const fsi_lbm_force* force, int32_t phase, int32_t face_slot,
+                 void* stream);
 
 /* fsi/lbm3d.py:499-534. op: 0 clear, 1 set (fluid = value, solid = 0; used by
  * reset_force and set_uniform_force), 2 add (fluid += value, solid = 0),
@@ -189,6 +200,14 @@ int fsi_mpm_p2g(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_cou
  * c->contact_enabled. */
 int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
                      const fsi_coupling_params* c, const fsi_coupling_buffers* cb, void* stream);
+
+/* The substep loop of LBMMpmCoupler.step (fsi/coupling.py:147-160) and of
+ * repeated MPMSolver3D.substep calls: `substeps` x (clear grid, P2G with the
+ * optional fused coupling, grid update + G2P with optional contact), one call.
+ * The occupancy phi is scattered on the first substep only when build_first. */
+int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
+                     const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
+                     int32_t substeps, int32_t build_first, int32_t clear_force, void* stream);
 
 /* fsi/coupling.py:463-496 alone (used by compute_coupling_forces). */
 int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,

@@ -102,6 +102,22 @@ def main():
         lines.append(f"constexpr long long {name}_NUM{shape} = {{" + ", ".join(str(v.numerator) for v in flat) + "};")
         lines.append(f"constexpr long long {name}_DEN{shape} = {{" + ", ".join(str(v.denominator) for v in flat) + "};")
 
+    # M^-1 = M^T (M M^T)^-1; the Gram matrix M M^T only couples moments of the
+    # same parity and order, so its inverse is block diagonal and cheap to apply
+    gram = [[sum(M[a][q] * M[b][q] for q in range(N)) for b in range(N)] for a in range(N)]
+    ginv = exact_inverse(gram)
+    for a in range(N):
+        for b in range(N):
+            assert sum(M[q][a] * ginv[q][b] for q in range(N)) == inv[a][b]
+    lines.append("// inverse Gram matrix (M M^T)^-1, block diagonal: M^-1 = M^T GINV")
+    lines.append("constexpr long long GINV_NUM[19][19] = {")
+    for row in ginv:
+        lines.append("    {" + ", ".join(str(v.numerator) for v in row) + "},")
+    lines.append("};")
+    lines.append("constexpr long long GINV_DEN[19][19] = {")
+    for row in ginv:
+        lines.append("    {" + ", ".join(str(v.denominator) for v in row) + "},")
+    lines.append("};")
     lines.append("// Guo moment tables, flattened: GA[k*3+a], GB[k], GC[(k*3+a)*3+b]")
     frac_table("GA", [ga[k][a] for k in range(N) for a in range(3)], "[57]")
     frac_table("GB", gb, "[19]")

@@ -2,20 +2,33 @@
 //
 // State layout in HBM (SoA, one plane per population):
 //   pop[q * N + n], n = (i*ny + j)*nz + k   (k fastest, as the reference)
-// so a wavefront of 64 consecutive cells reads and writes 64 consecutive
-// entries of each population plane.
+// so a wavefront of 64 consecutive cells touches 64 consecutive entries of
+// every population plane.
 //
-// One step is one fused kernel (collide + push-stream with half-way
-// bounce-back) plus a small x-face boundary kernel. The populations a cell
-// reads are its own, so the reads are perfectly coalesced; the pushes go to
-// the 18 neighbours and are coalesced along k.
+// Pull scheme. Between steps the lattice is in one of two states:
+//   post: pop holds post-collision populations f* of the last step;
+//   mat:  pop holds the reference's post-stream/post-boundary populations f,
+//         and rho/vel hold the macroscopic fields (fsi/lbm3d.py:409-425).
+// One kernel, k_lbm_pull, serves three phases:
+//   STREAM_COLLIDE (post -> post): gather f* from the neighbours (half-way
+//       bounce-back from the cell's own opposite slot when the neighbour is
+//       solid), x-face boundaries, macroscopic fields, collision with the
+//       current force, write the cell's own 19 slots;
+//   STREAM_MACRO (post -> mat): the same gather and boundaries, then write f,
+//       rho and u (accessors, the coupler, and any force change need them);
+//   COLLIDE (mat -> post): collide the stored f with the stored rho/u.
+// Every lane, solid ones included, executes the same instruction stream and
+// writes only its own cell's slots (solid cells rewrite their constant rest
+// populations): exec-mask divergence around memory instructions was measured
+// to halve the achieved bandwidth on porous masks.
 //
 // Collision: f* = F + M^-1 r with r_k = -S_k (m_k - meq_k) + (1 - S_k/2) G_k,
 // algebraically the reference's m = M F; m -= S (m - meq); m += (1 - S/2) G;
 // f = M^-1 m (fsi/lbm3d.py:314-333). The moments use the even/odd structure of
 // M under e -> -e (pair sums and differences of opposite populations), the
-// Guo moments G_k use their closed form in (u, F), and M^-1 is the exact
-// rational inverse; conserved moments (S_k = 0) keep r_k = G_k.
+// Guo moments G_k their closed form in (u, F), and M^-1 = M^T (M M^T)^-1 with
+// the block-diagonal exact Gram inverse; conserved moments (S_k = 0) keep
+// r_k = G_k.
 
 #include "common.h"
 #include "mrt_tables.h"
@@ -33,10 +46,14 @@ template <typename R> struct LbmK {
   int nx, ny, nz;
   int64_t N;
   R s_v, s_o;
+  R cv, co;     // 1 - s/2 for the two relaxation rates
   R rho0;
   R rho_base;   // rho0 for shifted storage, 0 for absolute storage
   R scale, inv_scale;
   R uf[3];      // uniform force when no field is given
+  int mode[2];  // x-face boundary modes (left, right)
+  R bc_rho[2];
+  R bc_vel[2][3];
 };
 
 template <typename R> LbmK<R> make_consts(const fsi_lbm_params* p, const fsi_lbm_force* f) {
@@ -47,12 +64,20 @@ template <typename R> LbmK<R> make_consts(const fsi_lbm_params* p, const fsi_lbm
   c.N = (int64_t)p->nx * p->ny * p->nz;
   c.s_v = R(p->s_v);
   c.s_o = R(p->s_o);
+  c.cv = R(1) - R(0.5) * c.s_v;
+  c.co = R(1) - R(0.5) * c.s_o;
   c.rho0 = R(p->rho0);
   const bool shifted = p->storage == FSI_POP_F16;
   c.rho_base = shifted ? R(p->rho0) : R(0);
   c.scale = shifted ? R(p->pop_scale) : R(1);
   c.inv_scale = shifted ? R(1.0 / p->pop_scale) : R(1);
   for (int a = 0; a < 3; ++a) c.uf[a] = f ? R(f->uniform[a]) : R(0);
+  c.mode[0] = p->left_mode;
+  c.mode[1] = p->right_mode;
+  for (int sd = 0; sd < 2; ++sd) {
+    c.bc_rho[sd] = R(p->bc_rho[sd]);
+    for (int a = 0; a < 3; ++a) c.bc_vel[sd][a] = R(p->bc_vel[sd][a]);
+  }
   return c;
 }
 
@@ -76,18 +101,13 @@ template <typename R> __device__ __forceinline__ R frac(long long num, long long
   return R(num) / R(den);
 }
 
-// MRT collision in place on d[19] (absolute or deviation populations).
+// MRT collision in place on d[19] (absolute or deviation populations), given
+// the opposite-pair sums ps[p] = d[2p+1] + d[2p+2] and differences pd[p].
+// cv/co are 1 - s_v/2 and 1 - s_o/2.
 template <typename R>
-__device__ __forceinline__ void mrt_collide(R d[19], R rho, const R u[3], const R fo[3], R s_v,
-                                            R s_o) {
+__device__ __forceinline__ void mrt_collide(R d[19], const R ps[9], const R pd[9], const R u[3],
+                                            const R fo[3], R s_v, R s_o, R cv, R co) {
   using namespace fsi_mrt;
-  // opposite pairs are (2p+1, 2p+2)
-  R ps[9], pd[9];
-#pragma unroll
-  for (int p = 0; p < 9; ++p) {
-    ps[p] = d[2 * p + 1] + d[2 * p + 2];
-    pd[p] = d[2 * p + 1] - d[2 * p + 2];
-  }
   const R uf = u[0] * fo[0] + u[1] * fo[1] + u[2] * fo[2];
   const R uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
   R r[19];
@@ -132,16 +152,24 @@ __device__ __forceinline__ void mrt_collide(R d[19], R rho, const R u[3], const 
       case 15: me = u[0] * u[2]; break;
       default: break;
     }
-    const R s = rate_kind(k) == 1 ? s_v : s_o;
-    r[k] = -s * (m - me) + (R(1) - R(0.5) * s) * g;
+    const bool sv = rate_kind(k) == 1;
+    r[k] = (sv ? cv : co) * g - (sv ? s_v : s_o) * (m - me);
   }
-  (void)rho;
-  // d += M^-1 r, by even/odd parts per opposite pair
+  // M^-1 r = M^T (M M^T)^-1 r: block-diagonal Gram inverse, then integer M^T
+  R y[19];
+#pragma unroll
+  for (int a = 0; a < 19; ++a) {
+    R acc = R(0);
+#pragma unroll
+    for (int b = 0; b < 19; ++b)
+      if (GINV_NUM[a][b] != 0) acc += frac<R>(GINV_NUM[a][b], GINV_DEN[a][b]) * r[b];
+    y[a] = acc;
+  }
   {
     R e0 = R(0);
 #pragma unroll
     for (int k = 0; k < 19; ++k)
-      if (!kOdd[k] && INV_NUM[0][k] != 0) e0 += frac<R>(INV_NUM[0][k], INV_DEN[0][k]) * r[k];
+      if (!kOdd[k] && M[k][0] != 0) e0 += R(M[k][0]) * y[k];
     d[0] += e0;
   }
 #pragma unroll
@@ -150,16 +178,29 @@ __device__ __forceinline__ void mrt_collide(R d[19], R rho, const R u[3], const 
     R ev = R(0), od = R(0);
 #pragma unroll
     for (int k = 0; k < 19; ++k) {
-      if (INV_NUM[q][k] == 0) continue;
-      const R c = frac<R>(INV_NUM[q][k], INV_DEN[q][k]);
+      if (M[k][q] == 0) continue;
       if (kOdd[k])
-        od += c * r[k];
+        od += R(M[k][q]) * y[k];
       else
-        ev += c * r[k];
+        ev += R(M[k][q]) * y[k];
     }
     d[q] += ev + od;
     d[q + 1] += ev - od;
   }
+}
+
+template <typename R> __device__ __forceinline__ R recip(R x) { return R(1) / x; }
+template <> __device__ __forceinline__ float recip<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);  // v_rcp_f32, 1 ulp
+}
+
+// 32-bit byte offsets from a uniform base: the compiler emits SGPR-base +
+// VGPR-offset addressing instead of 64-bit per-lane address arithmetic.
+template <typename T> __device__ __forceinline__ T ld_off(const T* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <typename T> __device__ __forceinline__ void st_off(T* base, uint32_t byte_off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 
 struct Nbr {
@@ -193,6 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_lbm_init(LbmK<typename PopStore<S>::
                                                      typename PopStore<S>::real* __restrict__ rho,
                                                      typename PopStore<S>::real* __restrict__ vel,
                                                      S* __restrict__ pa, S* __restrict__ pb,
+                                                     typename PopStore<S>::real* __restrict__ face_vel,
                                                      typename PopStore<S>::real u0x,
                                                      typename PopStore<S>::real u0y,
                                                      typename PopStore<S>::real u0z) {
@@ -215,6 +257,13 @@ __global__ __launch_bounds__(kBlock) void k_lbm_init(LbmK<typename PopStore<S>::
   vel[n] = u[0];
   vel[c.N + n] = u[1];
   vel[2 * c.N + n] = u[2];
+  if (face_vel && (i == 0 || i == c.nx - 1)) {
+    // slot 0 holds the velocity the first step's pressure faces read
+    const int64_t plane = (int64_t)c.ny * c.nz;
+    R* fv = face_vel + (i == 0 ? 0 : 3 * plane);
+    const int64_t jk = (int64_t)j * c.nz + k;
+    for (int a = 0; a < 3; ++a) fv[a * plane + jk] = u[a];
+  }
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     const R v = feq_dev<R, PopStore<S>::shifted>(q, c.rho0, c.rho0, u);
@@ -224,168 +273,96 @@ __global__ __launch_bounds__(kBlock) void k_lbm_init(LbmK<typename PopStore<S>::
   }
 }
 
-// Fused collide + push stream (fsi/lbm3d.py:314-361) for one fluid cell.
-template <typename S, bool FORCE_FIELD, bool MACRO_IN, bool FACE_VEL>
-__global__ __launch_bounds__(kBlock) void k_lbm_step(
-    LbmK<typename PopStore<S>::real> c, const S* __restrict__ pin, S* __restrict__ pout,
-    const uint32_t* __restrict__ nbmask, const typename PopStore<S>::real* __restrict__ force,
-    const typename PopStore<S>::real* __restrict__ rho_in,
-    const typename PopStore<S>::real* __restrict__ vel_in,
-    typename PopStore<S>::real* __restrict__ face_vel) {
+enum { PHASE_STREAM_COLLIDE = 0, PHASE_COLLIDE = 1, PHASE_STREAM_MACRO = 2 };
+
+// Rest population of a solid cell in storage units: w_q rho0 (absolute) or 0
+// (shifted). Solid cells keep it for ever (the reference never updates them).
+template <typename S>
+__device__ __forceinline__ S solid_value(int q, typename PopStore<S>::real rho0,
+                                         typename PopStore<S>::real scale) {
   using R = typename PopStore<S>::real;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= c.N) return;
-  const uint32_t mask = nbmask[n];
-  if (mask & 1u) return;  // solid cells are never collided or streamed
-  const int64_t N = c.N;
-
-  R d[19];
-#pragma unroll
-  for (int q = 0; q < 19; ++q) d[q] = PopStore<S>::load(pin[q * N + n], c.inv_scale);
-
-  R fo[3];
-  if constexpr (FORCE_FIELD) {
-    fo[0] = force[n];
-    fo[1] = force[N + n];
-    fo[2] = force[2 * N + n];
-  } else {
-    fo[0] = c.uf[0];
-    fo[1] = c.uf[1];
-    fo[2] = c.uf[2];
-  }
-
-  // macroscopic state of the previous step (fsi/lbm3d.py:409-425)
-  R rho, u[3];
-  if constexpr (MACRO_IN) {
-    rho = rho_in[n];
-    u[0] = vel_in[n];
-    u[1] = vel_in[N + n];
-    u[2] = vel_in[2 * N + n];
-  } else {
-    R sum = R(0), jx = R(0), jy = R(0), jz = R(0);
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      sum += d[q];
-      if (ex(q)) jx += R(ex(q)) * d[q];
-      if (ey(q)) jy += R(ey(q)) * d[q];
-      if (ez(q)) jz += R(ez(q)) * d[q];
-    }
-    rho = c.rho_base + sum;
-    const R inv = R(1) / rho;
-    u[0] = jx * inv + (fo[0] * R(0.5)) * inv;
-    u[1] = jy * inv + (fo[1] * R(0.5)) * inv;
-    u[2] = jz * inv + (fo[2] * R(0.5)) * inv;
-  }
-
-  const int k = (int)(n % c.nz);
-  const int64_t ij = n / c.nz;
-  const int j = (int)(ij % c.ny);
-  const int i = (int)(ij / c.ny);
-  if constexpr (FACE_VEL) {
-    if (i == 0 || i == c.nx - 1) {
-      const int64_t plane = (int64_t)c.ny * c.nz;
-      R* fv = face_vel + (i == 0 ? 0 : 3 * plane);
-      const int64_t jk = (int64_t)j * c.nz + k;
-      fv[jk] = u[0];
-      fv[plane + jk] = u[1];
-      fv[2 * plane + jk] = u[2];
-    }
-  }
-
-  mrt_collide<R>(d, rho, u, fo, c.s_v, c.s_o);
-
-  const Nbr o = neighbours(i, j, k, c.nx, c.ny, c.nz);
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const S s = PopStore<S>::store(d[q], c.scale);
-    if (q == 0) {
-      pout[n] = s;
-    } else if (mask & (1u << q)) {
-      pout[opp(q) * N + n] = s;  // half-way bounce-back into the source cell
-    } else {
-      pout[q * N + nbr_index(n, o, q)] = s;
-    }
-  }
+  return PopStore<S>::store(PopStore<S>::shifted ? R(0) : wq<R>(q) * rho0, scale);
 }
 
-// x-face boundaries on the streamed populations, fsi/lbm3d.py:363-407.
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_lbm_bc(LbmK<typename PopStore<S>::real> c,
-                                                   S* __restrict__ pop,
-                                                   const int32_t* __restrict__ solid,
-                                                   const typename PopStore<S>::real* __restrict__ face_vel,
-                                                   int side, int mode,
-                                                   typename PopStore<S>::real rho_bc,
-                                                   typename PopStore<S>::real vx,
-                                                   typename PopStore<S>::real vy,
-                                                   typename PopStore<S>::real vz) {
-  using R = typename PopStore<S>::real;
-  constexpr bool SH = PopStore<S>::shifted;
-  const int64_t plane = (int64_t)c.ny * c.nz;
-  const int64_t jk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (jk >= plane) return;
-  const int face = side == 0 ? 0 : c.nx - 1;
-  const int inner = side == 0 ? 1 : c.nx - 2;
-  const int64_t n = (int64_t)face * plane + jk;
-  if (solid[n] != 0) return;
-  const int64_t N = c.N;
+// x-face boundary of one face cell on its streamed populations d[19]
+// (fsi/lbm3d.py:363-407). prev_u: the face cell's macroscopic velocity of the
+// previous step; inner_solid: the inner neighbour is solid (its velocity is 0).
+template <typename R, bool SH>
+__device__ __forceinline__ void x_face_boundary(R d[19], int mode, R rho_bc, const R ubc[3],
+                                                R rho0, const R prev_u[3], bool inner_solid) {
   if (mode == FSI_BC_PRESSURE) {
-    // velocity of the face cell from the previous macroscopic state, or of the
-    // inner neighbour when that one is solid (a solid cell's velocity is 0)
-    R u[3] = {R(0), R(0), R(0)};
-    if (solid[(int64_t)inner * plane + jk] == 0) {
-      const R* fv = face_vel + side * 3 * plane;
-      u[0] = fv[jk];
-      u[1] = fv[plane + jk];
-      u[2] = fv[2 * plane + jk];
-    }
+    R u[3] = {prev_u[0], prev_u[1], prev_u[2]};
+    if (inner_solid) u[0] = u[1] = u[2] = R(0);
 #pragma unroll
-    for (int q = 0; q < 19; ++q)
-      pop[q * N + n] = PopStore<S>::store(feq_dev<R, SH>(q, rho_bc, c.rho0, u), c.scale);
-  } else {
-    // velocity: sequential in-place F_q = feq(opp q; rho0, u_bc) - F_opp + feq(q; rho0, u_bc)
-    const R ub[3] = {vx, vy, vz};
-    R d[19];
-#pragma unroll
-    for (int q = 0; q < 19; ++q) d[q] = PopStore<S>::load(pop[q * N + n], c.inv_scale);
+    for (int q = 0; q < 19; ++q) d[q] = feq_dev<R, SH>(q, rho_bc, rho0, u);
+  } else if (mode == FSI_BC_VELOCITY) {
+    // sequential in-place F_q = feq(opp q; rho0, u_bc) - F_opp + feq(q; rho0, u_bc)
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
       const int o = opp(q);
-      d[q] = feq_dev<R, SH>(o, c.rho0, c.rho0, ub) - d[o] + feq_dev<R, SH>(q, c.rho0, c.rho0, ub);
+      d[q] = feq_dev<R, SH>(o, rho0, rho0, ubc) - d[o] + feq_dev<R, SH>(q, rho0, rho0, ubc);
     }
-#pragma unroll
-    for (int q = 0; q < 19; ++q) pop[q * N + n] = PopStore<S>::store(d[q], c.scale);
   }
 }
 
-// fsi/lbm3d.py:409-425 (update_macro) without the f = F copy.
-template <typename S, bool FORCE_FIELD>
-__global__ __launch_bounds__(kBlock) void k_lbm_macro(LbmK<typename PopStore<S>::real> c,
-                                                      const S* __restrict__ pop,
-                                                      const int32_t* __restrict__ solid,
-                                                      const typename PopStore<S>::real* __restrict__ force,
-                                                      typename PopStore<S>::real* __restrict__ rho,
-                                                      typename PopStore<S>::real* __restrict__ vel) {
+// One lattice update of one cell in the given phase (see the file comment).
+// FACES: some x face is not periodic. Requires 19 * N * sizeof(S) < 2^32.
+template <typename S, int PHASE, bool FORCE_FIELD, bool FACES>
+__global__ __launch_bounds__(kBlock) void k_lbm_pull(
+    LbmK<typename PopStore<S>::real> c, const S* __restrict__ pin, S* __restrict__ pout,
+    const uint32_t* __restrict__ nbmask, const typename PopStore<S>::real* __restrict__ force,
+    typename PopStore<S>::real* __restrict__ rho_io, typename PopStore<S>::real* __restrict__ vel_io,
+    const typename PopStore<S>::real* __restrict__ fv_prev,
+    typename PopStore<S>::real* __restrict__ fv_next) {
   using R = typename PopStore<S>::real;
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= c.N) return;
-  const int64_t N = c.N;
-  if (solid[n] != 0) {
-    rho[n] = c.rho0;
-    vel[n] = R(0);
-    vel[N + n] = R(0);
-    vel[2 * N + n] = R(0);
-    return;
-  }
-  R sum = R(0), jx = R(0), jy = R(0), jz = R(0);
+  constexpr bool SH = PopStore<S>::shifted;
+  const uint32_t N = (uint32_t)c.N;
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  constexpr uint32_t B = sizeof(S);
+  const uint32_t plane = N * B;  // byte size of one population plane
+  const uint32_t own = n * B;
+  const uint32_t mask = nbmask[n];
+  const bool solid = (mask & 1u) != 0;
+
+  const uint32_t nz = (uint32_t)c.nz, ny = (uint32_t)c.ny, nx = (uint32_t)c.nx;
+  const uint32_t k = n % nz;
+  const uint32_t ij = n / nz;
+  const uint32_t j = ij % ny;
+  const uint32_t i = ij / ny;
+
+  R d[19];
+  if constexpr (PHASE == PHASE_COLLIDE) {
 #pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const R dq = PopStore<S>::load(pop[q * N + n], c.inv_scale);
-    sum += dq;
-    if (ex(q)) jx += R(ex(q)) * dq;
-    if (ey(q)) jy += R(ey(q)) * dq;
-    if (ez(q)) jz += R(ez(q)) * dq;
+    for (int q = 0; q < 19; ++q) d[q] = PopStore<S>::load(ld_off(pin, q * plane + own), c.inv_scale);
+  } else {
+    // gather f*(c - e_q, q), or f*(c, opp q) when c - e_q = c + e_opp(q) is
+    // solid (fsi/lbm3d.py:352-361 read backwards); a solid lane reads its own
+    // slots. One-cell periodic wrap on every axis (lbm3d.py:335-350).
+    const uint32_t sx = ny * nz;
+    const uint32_t oxm = i == 0 ? (nx - 1) * sx : (uint32_t)(-(int32_t)sx);
+    const uint32_t oxp = i == nx - 1 ? (uint32_t)(-(int32_t)((nx - 1) * sx)) : sx;
+    const uint32_t oym = j == 0 ? (ny - 1) * nz : (uint32_t)(-(int32_t)nz);
+    const uint32_t oyp = j == ny - 1 ? (uint32_t)(-(int32_t)((ny - 1) * nz)) : nz;
+    const uint32_t ozm = k == 0 ? nz - 1 : (uint32_t)(-1);
+    const uint32_t ozp = k == nz - 1 ? (uint32_t)(-(int32_t)(nz - 1)) : 1u;
+    const uint32_t all = solid ? ~0u : mask;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      uint32_t src = n;  // c - e_q
+      if (ex(q) > 0) src += oxm;
+      if (ex(q) < 0) src += oxp;
+      if (ey(q) > 0) src += oym;
+      if (ey(q) < 0) src += oyp;
+      if (ez(q) > 0) src += ozm;
+      if (ez(q) < 0) src += ozp;
+      const uint32_t a0 = q * plane + src * B;
+      const uint32_t a1 = opp(q) * plane + own;
+      const uint32_t sel = 0u - ((all >> opp(q)) & 1u);
+      d[q] = PopStore<S>::load(ld_off(pin, a0 ^ ((a0 ^ a1) & sel)), c.inv_scale);
+    }
   }
+
   R fo[3];
   if constexpr (FORCE_FIELD) {
     fo[0] = force[n];
@@ -396,12 +373,88 @@ __global__ __launch_bounds__(kBlock) void k_lbm_macro(LbmK<typename PopStore<S>:
     fo[1] = c.uf[1];
     fo[2] = c.uf[2];
   }
-  const R r = c.rho_base + sum;
-  const R inv = R(1) / r;
-  rho[n] = r;
-  vel[n] = jx * inv + (fo[0] * R(0.5)) * inv;
-  vel[N + n] = jy * inv + (fo[1] * R(0.5)) * inv;
-  vel[2 * N + n] = jz * inv + (fo[2] * R(0.5)) * inv;
+
+  bool face = false;
+  int side = 0;
+  uint32_t jk = 0;
+  const uint32_t fplane = ny * nz;
+  if constexpr (FACES && PHASE != PHASE_COLLIDE) {
+    // x-face boundaries of the streamed populations, with the previous
+    // macroscopic velocity of the face cell
+    side = i == 0 ? 0 : 1;
+    face = (i == 0 && c.mode[0] != FSI_BC_PERIODIC) || (i == nx - 1 && c.mode[1] != FSI_BC_PERIODIC);
+    jk = j * nz + k;
+    if (face && !solid) {
+      const R* fp = fv_prev + side * 3 * fplane;
+      const R pu[3] = {fp[jk], fp[fplane + jk], fp[2 * fplane + jk]};
+      const bool inner_solid = (mask >> (side == 0 ? 1 : 2)) & 1u;
+      x_face_boundary<R, SH>(d, c.mode[side], c.bc_rho[side], c.bc_vel[side], c.rho0, pu, inner_solid);
+    }
+  }
+
+  R ps[9], pd[9];
+#pragma unroll
+  for (int p = 0; p < 9; ++p) {
+    ps[p] = d[2 * p + 1] + d[2 * p + 2];
+    pd[p] = d[2 * p + 1] - d[2 * p + 2];
+  }
+
+  // macroscopic state (fsi/lbm3d.py:409-425): stored, or from the streamed
+  // populations with the force the previous collision used
+  R rho, u[3];
+  if constexpr (PHASE == PHASE_COLLIDE) {
+    rho = rho_io[n];
+    u[0] = vel_io[n];
+    u[1] = vel_io[N + n];
+    u[2] = vel_io[2 * N + n];
+  } else {
+    R sum = d[0];
+    R jm[3] = {R(0), R(0), R(0)};
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      sum += ps[p];
+      const int q = 2 * p + 1;
+      if (ex(q)) jm[0] += R(ex(q)) * pd[p];
+      if (ey(q)) jm[1] += R(ey(q)) * pd[p];
+      if (ez(q)) jm[2] += R(ez(q)) * pd[p];
+    }
+    rho = c.rho_base + sum;
+    const R inv = recip(rho);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) u[a] = (jm[a] + fo[a] * R(0.5)) * inv;
+    if (solid) {
+      rho = c.rho0;
+      u[0] = u[1] = u[2] = R(0);
+    }
+    if constexpr (FACES) {
+      if (face) {
+        R* fn = fv_next + side * 3 * fplane;
+        fn[jk] = u[0];
+        fn[fplane + jk] = u[1];
+        fn[2 * fplane + jk] = u[2];
+      }
+    }
+  }
+
+  if constexpr (PHASE == PHASE_STREAM_MACRO) {
+    rho_io[n] = rho;
+    vel_io[n] = u[0];
+    vel_io[N + n] = u[1];
+    vel_io[2 * N + n] = u[2];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      const S v = solid ? solid_value<S>(q, c.rho0, c.scale) : PopStore<S>::store(d[q], c.scale);
+      st_off(pout, q * plane + own, v);
+    }
+    return;
+  }
+
+  mrt_collide<R>(d, ps, pd, u, fo, c.s_v, c.s_o, c.cv, c.co);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const S v = solid ? solid_value<S>(q, c.rho0, c.scale) : PopStore<S>::store(d[q], c.scale);
+    st_off(pout, q * plane + own, v);
+  }
 }
 
 // fsi/lbm3d.py:499-534
@@ -500,71 +553,60 @@ int lbm_init_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pa, void
   using R = typename PopStore<S>::real;
   LbmK<R> c = make_consts<R>(p, nullptr);
   hipLaunchKernelGGL(k_lbm_init<S>, dim3(blocks_for(c.N)), dim3(kBlock), 0, st, c, b->solid,
-                     b->nbmask, (R*)b->rho, (R*)b->vel, (S*)pa, (S*)pb,
+                     b->nbmask, (R*)b->rho, (R*)b->vel, (S*)pa, (S*)pb, (R*)b->face_vel,
                      R(p->initial_velocity[0]), R(p->initial_velocity[1]),
                      R(p->initial_velocity[2]));
   FSI_TRY_LAUNCH("lbm_init");
   return 0;
 }
 
-template <typename S, bool FF, bool MI>
-void launch_step(const LbmK<typename PopStore<S>::real>& c, bool face, const S* pin, S* pout,
-                 const fsi_lbm_buffers* b, const void* force, hipStream_t st) {
+template <typename S, int PHASE, bool FF>
+void launch_pull(const LbmK<typename PopStore<S>::real>& c, bool faces, const void* pin, void* pout,
+                 const fsi_lbm_buffers* b, const void* force, const void* fv_prev, void* fv_next,
+                 hipStream_t st) {
   using R = typename PopStore<S>::real;
-  if (face)
-    hipLaunchKernelGGL((k_lbm_step<S, FF, MI, true>), dim3(blocks_for(c.N)), dim3(kBlock), 0, st,
-                       c, pin, pout, b->nbmask, (const R*)force, (const R*)b->rho,
-                       (const R*)b->vel, (R*)b->face_vel);
+  const dim3 grid(blocks_for(c.N)), block(kBlock);
+  if (faces)
+    hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, true>), grid, block, 0, st, c, (const S*)pin,
+                       (S*)pout, b->nbmask, (const R*)force, (R*)b->rho, (R*)b->vel,
+                       (const R*)fv_prev, (R*)fv_next);
   else
-    hipLaunchKernelGGL((k_lbm_step<S, FF, MI, false>), dim3(blocks_for(c.N)), dim3(kBlock), 0, st,
-                       c, pin, pout, b->nbmask, (const R*)force, (const R*)b->rho,
-                       (const R*)b->vel, (R*)b->face_vel);
+    hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, false>), grid, block, 0, st, c, (const S*)pin,
+                       (S*)pout, b->nbmask, (const R*)force, (R*)b->rho, (R*)b->vel,
+                       (const R*)fv_prev, (R*)fv_next);
 }
 
 template <typename S>
 int lbm_step_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pin, void* pout,
-               const fsi_lbm_force* f, int macro_in, hipStream_t st) {
+               const fsi_lbm_force* f, int phase, int face_slot, hipStream_t st) {
   using R = typename PopStore<S>::real;
   LbmK<R> c = make_consts<R>(p, f);
+  FSI_REQUIRE(19.0 * (double)c.N * sizeof(S) < 4294967296.0,
+              "lbm_step: lattice of %lld cells too large for 32-bit plane offsets", (long long)c.N);
+  const bool faces = p->left_mode != FSI_BC_PERIODIC || p->right_mode != FSI_BC_PERIODIC;
+  FSI_REQUIRE(!faces || b->face_vel, "lbm_step: non-periodic x faces need the face velocity buffer");
+  FSI_REQUIRE(phase != FSI_PHASE_COLLIDE || (b->rho && b->vel), "lbm_step: collide needs rho and vel");
+  FSI_REQUIRE(phase != FSI_PHASE_STREAM_MACRO || (b->rho && b->vel), "lbm_step: macro needs rho and vel");
+  const int64_t fsize = 6LL * p->ny * p->nz;
+  const R* fvp = faces ? (const R*)b->face_vel + (face_slot ? fsize : 0) : nullptr;
+  R* fvn = faces ? (R*)b->face_vel + (face_slot ? 0 : fsize) : nullptr;
   const bool field = f && f->field;
-  const bool face = p->left_mode == FSI_BC_PRESSURE || p->right_mode == FSI_BC_PRESSURE;
-  FSI_REQUIRE(!face || b->face_vel, "pressure boundary needs a face velocity buffer");
-  const S* in = (const S*)pin;
-  S* out = (S*)pout;
   const void* fp = field ? f->field : nullptr;
-  if (field && macro_in)
-    launch_step<S, true, true>(c, face, in, out, b, fp, st);
-  else if (field)
-    launch_step<S, true, false>(c, face, in, out, b, fp, st);
-  else if (macro_in)
-    launch_step<S, false, true>(c, face, in, out, b, fp, st);
-  else
-    launch_step<S, false, false>(c, face, in, out, b, fp, st);
-  FSI_TRY_LAUNCH("lbm_step");
-  const int64_t plane = (int64_t)p->ny * p->nz;
-  for (int side = 0; side < 2; ++side) {
-    const int mode = side == 0 ? p->left_mode : p->right_mode;
-    if (mode == FSI_BC_PERIODIC) continue;
-    hipLaunchKernelGGL(k_lbm_bc<S>, dim3(blocks_for(plane)), dim3(kBlock), 0, st, c, out, b->solid,
-                       (const R*)b->face_vel, side, mode, R(p->bc_rho[side]),
-                       R(p->bc_vel[side][0]), R(p->bc_vel[side][1]), R(p->bc_vel[side][2]));
-    FSI_TRY_LAUNCH("lbm_bc");
+#define FSI_PULL(PH)                                                      \
+  do {                                                                    \
+    if (field)                                                            \
+      launch_pull<S, PH, true>(c, faces, pin, pout, b, fp, fvp, fvn, st); \
+    else                                                                  \
+      launch_pull<S, PH, false>(c, faces, pin, pout, b, fp, fvp, fvn, st); \
+  } while (0)
+  switch (phase) {
+    case FSI_PHASE_STREAM_COLLIDE: FSI_PULL(PHASE_STREAM_COLLIDE); break;
+    case FSI_PHASE_COLLIDE: FSI_PULL(PHASE_COLLIDE); break;
+    case FSI_PHASE_STREAM_MACRO: FSI_PULL(PHASE_STREAM_MACRO); break;
+    default: FSI_REQUIRE(false, "lbm_step: unknown phase %d", phase);
   }
-  return 0;
-}
-
-template <typename S>
-int lbm_macro_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop,
-                const fsi_lbm_force* f, hipStream_t st) {
-  using R = typename PopStore<S>::real;
-  LbmK<R> c = make_consts<R>(p, f);
-  if (f && f->field)
-    hipLaunchKernelGGL((k_lbm_macro<S, true>), dim3(blocks_for(c.N)), dim3(kBlock), 0, st, c,
-                       (const S*)pop, b->solid, (const R*)f->field, (R*)b->rho, (R*)b->vel);
-  else
-    hipLaunchKernelGGL((k_lbm_macro<S, false>), dim3(blocks_for(c.N)), dim3(kBlock), 0, st, c,
-                       (const S*)pop, b->solid, (const R*)nullptr, (R*)b->rho, (R*)b->vel);
-  FSI_TRY_LAUNCH("lbm_macro");
+#undef FSI_PULL
+  FSI_TRY_LAUNCH("lbm_step");
   return 0;
 }
 
@@ -599,28 +641,16 @@ int fsi_lbm_init(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pop_a,
 }
 
 int fsi_lbm_step(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop_in,
-                 void* pop_out, const fsi_lbm_force* force, int32_t macro_in, void* stream) {
+                 void* pop_out, const fsi_lbm_force* force, int32_t phase, int32_t face_slot,
+                 void* stream) {
   if (int rc = check_params(p)) return rc;
-  FSI_REQUIRE(b && b->solid && b->nbmask && pop_in && pop_out && pop_in != pop_out,
+  FSI_REQUIRE(b && b->nbmask && pop_in && pop_out && pop_in != pop_out,
               "lbm_step: null or aliased buffer");
-  FSI_REQUIRE(!macro_in || (b->rho && b->vel), "lbm_step: macro_in needs rho and vel");
   hipStream_t st = (hipStream_t)stream;
   switch (p->storage) {
-    case FSI_POP_F32: return lbm_step_t<float>(p, b, pop_in, pop_out, force, macro_in, st);
-    case FSI_POP_F16: return lbm_step_t<__half>(p, b, pop_in, pop_out, force, macro_in, st);
-    default: return lbm_step_t<double>(p, b, pop_in, pop_out, force, macro_in, st);
-  }
-}
-
-int fsi_lbm_macro(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop,
-                  const fsi_lbm_force* force, void* stream) {
-  if (int rc = check_params(p)) return rc;
-  FSI_REQUIRE(b && b->solid && b->rho && b->vel && pop, "lbm_macro: null buffer");
-  hipStream_t st = (hipStream_t)stream;
-  switch (p->storage) {
-    case FSI_POP_F32: return lbm_macro_t<float>(p, b, pop, force, st);
-    case FSI_POP_F16: return lbm_macro_t<__half>(p, b, pop, force, st);
-    default: return lbm_macro_t<double>(p, b, pop, force, st);
+    case FSI_POP_F32: return lbm_step_t<float>(p, b, pop_in, pop_out, force, phase, face_slot, st);
+    case FSI_POP_F16: return lbm_step_t<__half>(p, b, pop_in, pop_out, force, phase, face_slot, st);
+    default: return lbm_step_t<double>(p, b, pop_in, pop_out, force, phase, face_slot, st);
   }
 }
 

@@ -766,6 +766,21 @@ int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_cou
   return contact_t<float>(p, b, c, cb, st);
 }
 
+int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
+                     const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
+                     int32_t substeps, int32_t build_first, int32_t clear_force, void* stream) {
+  FSI_REQUIRE(substeps >= 0, "mpm_substeps: negative substep count");
+  fsi_coupling_params local{};
+  if (c) local = *c;
+  for (int s = 0; s < substeps; ++s) {
+    local.build_fraction = (s == 0 && build_first) ? 1 : 0;
+    if (int rc = fsi_mpm_clear_grid(p, b, stream)) return rc;
+    if (int rc = fsi_mpm_p2g(p, b, c ? &local : nullptr, cb, clear_force, 1, stream)) return rc;
+    if (int rc = fsi_mpm_grid_g2p(p, b, c ? &local : nullptr, cb, stream)) return rc;
+  }
+  return 0;
+}
+
 int fsi_polar_rotation(int32_t real, int32_t n, const void* F, void* R, void* stream) {
   FSI_REQUIRE(F && R && n >= 0, "polar_rotation: bad arguments");
   if (n == 0) return 0;

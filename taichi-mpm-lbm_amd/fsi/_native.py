@@ -16,6 +16,7 @@ _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libfsi_hip.so"
 _lib = None
 
 POP_F32, POP_F16, POP_F64 = 0, 1, 2
+PHASE_STREAM_COLLIDE, PHASE_COLLIDE, PHASE_STREAM_MACRO = 0, 1, 2
 REAL_F32, REAL_F64 = 0, 1
 BC_MODES = {"periodic": 0, "pressure": 1, "velocity": 2}
 STORAGE_CODES = {"fp32": POP_F32, "fp16": POP_F16, "fp64": POP_F64}
@@ -96,8 +97,7 @@ class ForceAssembly(ctypes.Structure):
 # name -> argument types; every entry point returns int
 _SIGNATURES = {
     "fsi_lbm_init": [_vp, _vp, _vp, _vp, _vp],
-    "fsi_lbm_step": [_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp],
-    "fsi_lbm_macro": [_vp, _vp, _vp, _vp, _vp],
+    "fsi_lbm_step": [_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp],
     "fsi_lbm_force_op": [_vp, _vp, ctypes.c_int32, _vp, _vp, _vp, _vp],
     "fsi_lbm_force_import": [_vp, _vp, _vp, _vp],
     "fsi_lbm_export_populations": [_vp, _vp, _vp, _vp],
@@ -106,6 +106,7 @@ _SIGNATURES = {
     "fsi_mpm_clear_grid": [_vp, _vp, _vp],
     "fsi_mpm_p2g": [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp],
     "fsi_mpm_grid_g2p": [_vp, _vp, _vp, _vp, _vp],
+    "fsi_mpm_substeps": [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp],
     "fsi_contact": [_vp, _vp, _vp, _vp, _vp],
     "fsi_polar_rotation": [ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp],
     "fsi_force_assemble": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp],

@@ -154,12 +154,10 @@ class LBMMpmCoupler:
         dt_ratio = mpm_dt / lbm_dt_value
         mp = self.mpm.params(self.mpm._dt_value(mpm_dt))
         mb = self.mpm._buffers
-        for substep in range(self.substeps):
-            build = substep == 0 and self.build_solid_volume_fraction
-            cp = self._cparams(dt_ratio, build, self.enabled)
-            nat.call("fsi_mpm_clear_grid", nat.ref(mp), nat.ref(mb))
-            nat.call("fsi_mpm_p2g", nat.ref(mp), nat.ref(mb), nat.ref(cp), nat.ref(self._cbuf), 1, 1)
-            nat.call("fsi_mpm_grid_g2p", nat.ref(mp), nat.ref(mb), nat.ref(cp), nat.ref(self._cbuf))
+        cp = self._cparams(dt_ratio, self.build_solid_volume_fraction, self.enabled)
+        # S x (clear grid, P2G + coupling, grid update + G2P + contact) in one call
+        nat.call("fsi_mpm_substeps", nat.ref(mp), nat.ref(mb), nat.ref(cp), nat.ref(self._cbuf),
+                 self.substeps, 1 if self.build_solid_volume_fraction else 0, 1)
 
         # clamp phi, reset the lattice force, add the coupling force and IB drag
         self.lbm._before_force_update()

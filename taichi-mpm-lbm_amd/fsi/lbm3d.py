@@ -4,22 +4,25 @@ Same constructor, methods, accessor shapes/dtypes and exceptions as the
 reference's ``LBMSolver3D`` (fsi/lbm3d.py:57-534). State lives in torch device
 tensors; every update is a HIP kernel of ``libfsi_hip.so``.
 
-State and its relation to the reference's fields:
+Between steps the lattice is in one of two states:
 
-* ``pop[cur]``  - populations after streaming and boundaries, the reference's
-  ``f`` (= ``F`` after ``update_macro``), SoA (19, N). FP16 storage keeps
-  (f - w rho0) * pop_scale.
-* ``rho``/``vel`` - the reference's macroscopic fields. They are materialised
-  lazily: a step does not write them; ``_ensure_macro`` runs the macro kernel
-  when an accessor, the coupler or a force update needs them.
-* ``force``     - the per-cell force field (SoA (3, N)). When the host knows
-  it is uniform on fluid cells (``_force_uniform``), the step kernel takes the
-  value as a constant and reads no force memory.
+* materialised - ``pop[cur]`` holds the reference's ``f`` (post-stream,
+  post-boundary populations, SoA (19, N)) and ``rho``/``vel`` hold its
+  macroscopic fields. ``initialize`` leaves this state.
+* post-collision - ``pop[cur]`` holds the populations right after the last
+  collision; streaming is deferred to the next kernel. ``step`` leaves this
+  state.
 
-The step kernel needs the previous step's (rho, u). It recomputes them from
-the populations with the current force, which is exact while the force has
-not changed since that step; the first force change after a step therefore
-materialises (rho, u) first, and the next step reads them (``macro_in``).
+A step from the post-collision state is one fused pull kernel (stream,
+x-face boundaries, macroscopic fields, collision). Anything that needs the
+reference's post-step state - the accessors, the coupler, and any change of
+the force field (the macroscopic velocity must use the force of the step
+that produced it) - first materialises it; the next step then only collides.
+FP16 storage keeps (f - w rho0) * pop_scale.
+
+When the host knows the force is uniform on fluid cells
+(``_force_uniform``), kernels take the value as a constant and read no force
+memory.
 """
 
 from __future__ import annotations
@@ -91,14 +94,16 @@ class LBMSolver3D:
         self.force = torch.zeros(3 * n, dtype=rd, device=dev)
         self.solid = torch.zeros(n, dtype=torch.int32, device=dev)
         self.nbmask = torch.zeros(n, dtype=torch.int32, device=dev)
-        self.face_vel = torch.zeros(6 * self.ny * self.nz, dtype=rd, device=dev)
+        # two slots (previous / next step) x two faces x 3 components
+        self.face_vel = torch.zeros(12 * self.ny * self.nz, dtype=rd, device=dev)
 
         self._params = self._make_params()
         self._buffers = nat.LbmBuffers(
             nat.ptr(self.solid), nat.ptr(self.nbmask), nat.ptr(self.rho), nat.ptr(self.vel),
             nat.ptr(self.force), nat.ptr(self.face_vel),
         )
-        self._macro_valid = False
+        self._materialized = False
+        self._face_slot = 0
         self._force_dirty = False
         self._force_uniform: tuple[float, float, float] | None = None
         self._initialized = False
@@ -152,7 +157,8 @@ class LBMSolver3D:
         nat.call("fsi_lbm_init", nat.ref(self._params), nat.ref(self._buffers),
                  nat.ptr(self.pop[0]), nat.ptr(self.pop[1]))
         self._cur = 0
-        self._macro_valid = True
+        self._face_slot = 0
+        self._materialized = True
         self._force_dirty = False
         self._initialized = True
         self.reset_force()
@@ -188,28 +194,35 @@ class LBMSolver3D:
             f.uniform = nat.c_double3(0.0, 0.0, 0.0)
         return f
 
+    def _launch(self, phase: int) -> None:
+        f = self._force_arg()
+        nat.call("fsi_lbm_step", nat.ref(self._params), nat.ref(self._buffers),
+                 nat.ptr(self.pop[self._cur]), nat.ptr(self.pop[1 - self._cur]), nat.ref(f),
+                 phase, self._face_slot)
+        self._cur = 1 - self._cur
+        if phase != nat.PHASE_COLLIDE and self._faces:
+            self._face_slot = 1 - self._face_slot
+
+    @property
+    def _faces(self) -> bool:
+        return self.bc_x_left_mode != 0 or self.bc_x_right_mode != 0
+
     def step(self) -> None:
         # fsi/lbm3d.py:427-433
         if not self._initialized:
             raise RuntimeError("LBMSolver3D.initialize must be called before step().")
-        f = self._force_arg()
-        nat.call("fsi_lbm_step", nat.ref(self._params), nat.ref(self._buffers),
-                 nat.ptr(self.pop[self._cur]), nat.ptr(self.pop[1 - self._cur]), nat.ref(f),
-                 1 if self._macro_valid else 0)
-        self._cur = 1 - self._cur
-        self._macro_valid = False
+        self._launch(nat.PHASE_COLLIDE if self._materialized else nat.PHASE_STREAM_COLLIDE)
+        self._materialized = False
         self._force_dirty = False
 
     def _ensure_macro(self) -> None:
-        """Materialise rho/vel of the current populations (update_macro, lbm3d.py:409-425)."""
-        if self._macro_valid:
+        """Materialise the reference's post-step state: f, rho, u (lbm3d.py:409-425)."""
+        if self._materialized:
             return
         if self._force_dirty:  # cannot happen: force updates materialise first
-            raise RuntimeError("internal: macroscopic state lost after a force update")
-        f = self._force_arg()
-        nat.call("fsi_lbm_macro", nat.ref(self._params), nat.ref(self._buffers),
-                 nat.ptr(self.pop[self._cur]), nat.ref(f))
-        self._macro_valid = True
+            raise RuntimeError("internal: post-collision state with a changed force")
+        self._launch(nat.PHASE_STREAM_MACRO)
+        self._materialized = True
 
     def _before_force_update(self) -> None:
         if self._initialized:
@@ -306,6 +319,7 @@ class LBMSolver3D:
         return self.solid.cpu().numpy().reshape(self.nx, self.ny, self.nz)
 
     def distribution_numpy(self) -> np.ndarray:
+        self._ensure_macro()
         out = torch.empty(self.n_cells * _Q, dtype=self.real_dtype, device=self.device)
         nat.call("fsi_lbm_export_populations", nat.ref(self._params), nat.ptr(self.pop[self._cur]),
                  nat.ptr(out))

@@ -24,6 +24,17 @@ def rel_inf(a, b) -> float:
     return float(np.abs(a - b).max() / scale)
 
 
+def close(a, b, rel, abs_floor) -> tuple[bool, float, float]:
+    """max|a - b| <= rel * max|b| + abs_floor; returns (ok, rel error, abs error)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.size == 0:
+        return True, 0.0, 0.0
+    diff = float(np.abs(a - b).max())
+    scale = float(np.abs(b).max())
+    return diff <= rel * scale + abs_floor, (diff / scale if scale else diff), diff
+
+
 def lbm_fields(solver, rho0=1.0) -> dict:
     f = solver.distribution_numpy()
     return {

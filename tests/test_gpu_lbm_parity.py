@@ -3,7 +3,7 @@
 Gates, per field (populations as deviations f - w rho0, density, velocity),
 with rel = max|a - b| / max|b|:
 
-* FP64 build vs FP64 oracle: rel <= 1e-10. The FP64 instantiation runs the
+* FP64 build vs FP64 oracle: max|a - b| <= 1e-10 max|b| + 1e-13. The FP64 instantiation runs the
   same kernel code as FP32, so this pins the kernel logic (collision, push
   stream, bounce-back, x-face boundaries, force handling, macro fields).
 * FP32 build vs FP64 oracle: within twice the error the reference's own FP32
@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from parity import W, channel_config, lbm_fields, porous_mask, rel_inf, run_lbm
+from parity import W, channel_config, close, lbm_fields, porous_mask, rel_inf, run_lbm
 
 from cases import periodic_lbm_config
 
@@ -81,9 +81,10 @@ def test_lbm_fp64_build_matches_fp64_oracle(name, oracle_runs):
     got = _run(fsi.LBMSolver3D, SCENARIOS[name], storage="fp64")
     ref = oracle_runs[name]["f64"]
     for key in ("dev", "rho", "u"):
-        err = rel_inf(got[key], ref[key])
-        print(f"{name} fp64 {key}: rel {err:.3e}")
-        assert err <= 1e-10, (name, key, err)
+        # populations and density are O(1): 1e-13 absolute is ~1000 FP64 ulps
+        ok, err, diff = close(got[key], ref[key], 1e-10, 1e-13)
+        print(f"{name} fp64 {key}: rel {err:.3e} abs {diff:.3e}")
+        assert ok, (name, key, err, diff)
 
 
 @pytest.mark.parametrize("name", list(SCENARIOS))

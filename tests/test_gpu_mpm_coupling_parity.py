@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from parity import coupled_scene, jittered_cube, rel_inf, sim_fields
+from parity import close, coupled_scene, jittered_cube, rel_inf, sim_fields
 
 from cases import small_mpm_config
 
@@ -57,9 +57,9 @@ def test_mpm_fp64_matches_fp64_oracle(mpm_oracle):
     cfg, pos, vel = _spinning_cube()
     got = _run_mpm(fsi.MPMSolver3D(cfg, precision="fp64"), pos, vel, 60)
     for key in ("x", "v", "F"):
-        err = rel_inf(got[key], mpm_oracle["f64"][key])
-        print(f"mpm fp64 {key}: rel {err:.3e}")
-        assert err <= 1e-9, (key, err)
+        ok, err, diff = close(got[key], mpm_oracle["f64"][key], 1e-9, 1e-13)
+        print(f"mpm fp64 {key}: rel {err:.3e} abs {diff:.3e}")
+        assert ok, (key, err, diff)
 
 
 def test_mpm_fp32_within_reference_band(mpm_oracle):
@@ -150,9 +150,9 @@ def test_coupled_fp64_matches_fp64_oracle(name, coupled_oracle):
     got = _run_sim(fsi.FSISimulation, "fp64", **COUPLED[name])
     ref = coupled_oracle[name]["f64"]
     for key in FIELDS:
-        err = rel_inf(got[key], ref[key])
-        print(f"{name} fp64 {key}: rel {err:.3e}")
-        assert err <= 1e-9, (name, key, err)
+        ok, err, diff = close(got[key], ref[key], 1e-9, 1e-13)
+        print(f"{name} fp64 {key}: rel {err:.3e} abs {diff:.3e}")
+        assert ok, (name, key, err, diff)
     assert np.array_equal(got["cmask"], ref["cmask"])
     for k in ("unsupported_particle_count", "partial_support_particle_count",
               "clipped_particle_count", "ib_active_cell_count", "contact_candidate_count",
