@@ -117,6 +117,26 @@ __device__ __forceinline__ bool inside(int nx, int ny, int nz, int i, int j, int
   return (unsigned)i < (unsigned)nx && (unsigned)j < (unsigned)ny && (unsigned)k < (unsigned)nz;
 }
 
+// Node index for a gather that is always in bounds: nodes outside the
+// lattice map to node 0 and report in = false, so all 27 loads of a particle
+// can be issued together instead of one dependent round trip per node.
+__device__ __forceinline__ int64_t gather_node(int nx, int ny, int nz, int i, int j, int k, bool& in) {
+  in = inside(nx, ny, nz, i, j, k);
+  return in ? ((int64_t)i * ny + j) * nz + k : 0;
+}
+
+// stencil offset q = (i*3+j)*3+k of the 27 nodes, decoded without division
+__device__ __forceinline__ void stencil_ijk(int q, int& i, int& j, int& k) {
+  i = (q >= 9) + (q >= 18);
+  const int r = q - 9 * i;
+  j = (r >= 3) + (r >= 6);
+  k = r - 3 * j;
+}
+
+template <typename R> __device__ __forceinline__ R pick3(int i, R a, R b, R c) {
+  return i == 0 ? a : (i == 1 ? b : c);
+}
+
 // ------------------------------------------------------------- SVD ----
 //
 // Rotation factor R = U V^T of the rotation-variant SVD F = U S V^T (U, V
@@ -260,13 +280,44 @@ template <typename R> __device__ __forceinline__ R det3(const R m[9]) {
 
 // ------------------------------------------------------- P2G pass ----
 
+// grid clear (fsi/mpm3d.py:205-209) and, when given, the per-substep
+// particle-diagnostic counters (fsi/coupling.py:291-297)
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf) {
+__global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf,
+                                                  int32_t* __restrict__ counters) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) buf[t] = R(0);
+  if (counters && t < 5) counters[t] = 0;
+}
+
+// LDS tile of one workgroup: 8 values per node (grid momentum xyz + mass,
+// coupling force density xyz + phi), 64 KB, so two workgroups fit a CU.
+// Per-workgroup P2G accumulation tile in LDS. The words are FP64 for both
+// precisions: on gfx950 ds_add_f64 issues at the integer-atomic rate, while
+// ds_add_f32 is processed lane by lane (measured ~20x slower per wave
+// instruction), and the FP64 partial sums are rounded to FP32 once per
+// workgroup at the flush.
+struct Tile {
+  using acc = double;
+  static constexpr int kValues = 8;
+  static constexpr int kNodes = (64 * 1024) / (kValues * (int)sizeof(acc));
+};
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return v;
 }
 
 // fsi/coupling.py:332-406 (COUPLE) fused with fsi/mpm3d.py:234-266 (P2G).
+// The grid and coupling accumulations go through a per-workgroup LDS tile
+// (LDS float atomics) and are flushed with one global atomic per non-zero
+// tile entry; the flush skips nodes outside the lattice, which receive nothing.
 template <typename R, bool COUPLE, bool P2G>
 __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
                                                 R* __restrict__ v, const R* __restrict__ C,
@@ -279,6 +330,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
                                                 R* __restrict__ valid_weight,
                                                 int32_t* __restrict__ cmask,
                                                 int32_t* __restrict__ counters, int clear_force) {
+  __shared__ Tile::acc tile[Tile::kNodes * Tile::kValues];
+  __shared__ int red[6][kBlock / 64];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = p < m.num;
   const bool act = live && active[p] != 0;
@@ -287,17 +340,84 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
 
   bool partial = false, unsupported = false, clipped = false;
   R pf[3] = {R(0), R(0), R(0)};
-  R xp[3], vp[3], fx[3], w[3][3];
-  int base[3];
+  R xp[3] = {R(0), R(0), R(0)}, vp[3] = {R(0), R(0), R(0)}, fx[3] = {R(0), R(0), R(0)}, w[3][3];
+  R Cm[9], Fo[9], mp = R(0), vp_vol = R(0);
+  int base[3] = {0, 0, 0};
+  // every per-particle load is issued here, ahead of the node loops, so the
+  // wave waits on memory once
   if (act) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       xp[a] = x[a * cap + p];
       vp[a] = v[a * cap + p];
     }
+    mp = mass[p];
+    vp_vol = vol[p];
+    if constexpr (P2G) {
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        Cm[r] = C[r * cap + p];
+        Fo[r] = Fd[r * cap + p];
+      }
+    }
     locate(xp, m.inv_dx, base, fx);
-    bspline(fx, w);
   }
+  bspline(fx, w);
+
+  // bounding box of the workgroup's base cells -> tile of (box + 2) nodes
+  constexpr int kBig = 1 << 30;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int mn[3], mx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = wave_min(act ? base[a] : kBig);
+    mx[a] = wave_max(act ? base[a] : -kBig);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      red[a][wid] = mn[a];
+      red[3 + a][wid] = mx[a];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = red[a][0];
+    mx[a] = red[3 + a][0];
+#pragma unroll
+    for (int wv = 1; wv < kBlock / 64; ++wv) {
+      mn[a] = min(mn[a], red[a][wv]);
+      mx[a] = max(mx[a], red[3 + a][wv]);
+    }
+  }
+  const bool any = mn[0] <= mx[0];
+  const long long tx = any ? (long long)mx[0] - mn[0] + 3 : 0;
+  const long long ty = any ? (long long)mx[1] - mn[1] + 3 : 0;
+  const long long tz = any ? (long long)mx[2] - mn[2] + 3 : 0;
+  const long long tnodes = tx * ty * tz;
+  // scatter target of the workgroup: its LDS tile when the cells of its
+  // particles fit, otherwise the global grid/coupling buffers directly
+  const bool use_lds = any && tnodes <= Tile::kNodes;
+  const int x0 = mn[0], y0 = mn[1], z0 = mn[2];
+  const int tyi = use_lds ? (int)ty : 0, tzi = use_lds ? (int)tz : 0;
+  const int tvals = use_lds ? (int)tnodes * Tile::kValues : 0;
+  // LDS tile, slot-major (tile[slot * tnodes + node]) so neighbouring nodes
+  // of one slot fall in different banks; it spans base..base+2 of every
+  // particle of the workgroup, so the scatter needs no per-node bounds test
+  // (out-of-lattice entries are dropped at the flush)
+  const int tn_all = use_lds ? (int)tnodes : 0;
+#define FSI_TILE(ni, nj, nk, slot) \
+  tile[(slot) * tn_all + (((ni) - x0) * tyi + ((nj) - y0)) * tzi + ((nk) - z0)]
+#define FSI_GLOBAL(node, slot, val)                      \
+  do {                                                   \
+    if ((slot) < 4)                                      \
+      atomicAdd(grid + 4 * (node) + (slot), (val));      \
+    else                                                 \
+      atomicAdd(cforce + 4 * (node) + ((slot) - 4), (val)); \
+  } while (0)
+  for (int t = threadIdx.x; t < tvals; t += blockDim.x) tile[t] = Tile::acc(0);
+  __syncthreads();
 
   if constexpr (COUPLE) {
     if (live && !act) {
@@ -305,22 +425,43 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
       cmask[p] = 0;
     }
     if (act) {
-      R wsum = R(0), u[3] = {R(0), R(0), R(0)};
+      // load phase: the 27 nodes' solid flags and fluid velocities are all in
+      // flight before the first is used
+      int32_t sn[27];
+      R un[27][3];
+      uint32_t inmask = 0;
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-            if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
-            const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
-            if (solid[node] != 0) continue;
+            bool in;
+            const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
+            const int q = (i * 3 + j) * 3 + k;
+            sn[q] = solid[node];
+            un[q][0] = lbm_vel[node];
+            un[q][1] = lbm_vel[N + node];
+            un[q][2] = lbm_vel[2 * N + node];
+            inmask |= (uint32_t)in << q;
+          }
+      // support of the valid (in-lattice, fluid) nodes, q ascending
+      R wsum = R(0), u[3] = {R(0), R(0), R(0)};
+      uint32_t valid = 0;  // bit q = (i*3+j)*3+k
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const int q = (i * 3 + j) * 3 + k;
+            const bool ok = ((inmask >> q) & 1u) && sn[q] == 0;
+            valid |= (uint32_t)ok << q;
             const R wt = w[i][0] * w[j][1] * w[k][2];
-            wsum += wt;
-            u[0] += wt * lbm_vel[node];
-            u[1] += wt * lbm_vel[N + node];
-            u[2] += wt * lbm_vel[2 * N + node];
+            wsum += ok ? wt : R(0);
+            u[0] += ok ? wt * un[q][0] : R(0);
+            u[1] += ok ? wt * un[q][1] : R(0);
+            u[2] += ok ? wt * un[q][2] : R(0);
           }
       valid_weight[p] = wsum;
       if (wsum > cp.min_w) {
@@ -336,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
             clipped = true;
           }
         }
-        const R gm = cp.gamma * mass[p];
+        const R gm = cp.gamma * mp;
 #pragma unroll
         for (int a = 0; a < 3; ++a) pf[a] = gm * dv[a];
         if (cp.use_force) {
@@ -350,24 +491,45 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
         cmask[p] = 1;
         // equal and opposite reaction (and occupancy) spread on the valid nodes
         const R fscale = cp.dt_ratio / cp.cell_volume;
-        const R phi = vol[p] / cp.cell_volume;
+        const R phi = vp_vol / cp.cell_volume;
+        const R rf[3] = {-pf[0] * fscale, -pf[1] * fscale, -pf[2] * fscale};
+        if (use_lds) {
+          // each lane walks the 27 nodes from its own starting offset, so
+          // lanes sharing a cell add to different LDS words on each
+          // instruction (same-address LDS atomics serialise); invalid nodes
+          // add exact zeros
+          int q = lane % 27;
+          for (int o = 0; o < 27; ++o) {
+            int i, j, k;
+            stencil_ijk(q, i, j, k);
+            const bool ok = (valid >> q) & 1u;
+            const R wt = pick3(i, w[0][0], w[1][0], w[2][0]) * pick3(j, w[0][1], w[1][1], w[2][1]) *
+                         pick3(k, w[0][2], w[1][2], w[2][2]);
+            const R nv = ok ? wt / wsum : R(0);
+            const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+            atomicAdd(&FSI_TILE(ni, nj, nk, 4), nv * rf[0]);
+            atomicAdd(&FSI_TILE(ni, nj, nk, 5), nv * rf[1]);
+            atomicAdd(&FSI_TILE(ni, nj, nk, 6), nv * rf[2]);
+            if (cp.build) atomicAdd(&FSI_TILE(ni, nj, nk, 7), nv * phi);
+            q = q == 26 ? 0 : q + 1;
+          }
+        } else {
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+          for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j)
+            for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-              if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
-              const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
-              if (solid[node] != 0) continue;
-              const R nw = (w[i][0] * w[j][1] * w[k][2]) / wsum;
-              R* cf = cforce + 4 * node;
-              atomicAdd(cf + 0, -nw * pf[0] * fscale);
-              atomicAdd(cf + 1, -nw * pf[1] * fscale);
-              atomicAdd(cf + 2, -nw * pf[2] * fscale);
-              if (cp.build) atomicAdd(cf + 3, nw * phi);
-            }
+              for (int k = 0; k < 3; ++k) {
+                if (!((valid >> ((i * 3 + j) * 3 + k)) & 1u)) continue;
+                const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+                const R nw = (w[i][0] * w[j][1] * w[k][2]) / wsum;
+                const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+                FSI_GLOBAL(node, 4, nw * rf[0]);
+                FSI_GLOBAL(node, 5, nw * rf[1]);
+                FSI_GLOBAL(node, 6, nw * rf[2]);
+                if (cp.build) FSI_GLOBAL(node, 7, nw * phi);
+              }
+        }
       } else {
         unsupported = true;
         cmask[p] = 0;
@@ -393,95 +555,122 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
   }
 
   if constexpr (P2G) {
-    if (!act) return;
-    const R dt = m.dt;
-    R Cm[9], Fo[9], Fn[9];
+    if (act) {
+      const R dt = m.dt;
+      R Fn[9];
+      // F <- (I + dt C) F
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      Cm[r] = C[r * cap + p];
-      Fo[r] = Fd[r * cap + p];
-    }
-    // F <- (I + dt C) F
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+        for (int s2 = 0; s2 < 3; ++s2) {
+          R acc = R(0);
 #pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        R acc = R(0);
-#pragma unroll
-        for (int t = 0; t < 3; ++t) acc += ((r == t ? R(1) : R(0)) + dt * Cm[r * 3 + t]) * Fo[t * 3 + s];
-        Fn[r * 3 + s] = acc;
-      }
-#pragma unroll
-    for (int r = 0; r < 9; ++r) Fd[r * cap + p] = Fn[r];
-    const R J = det3(Fn);
-    R Rot[9];
-    rotation_factor<R>(Fn, Rot, J);
-    // P = 2 mu (F - R) F^T + lam J (J - 1) I; affine = -dt vol 4/dx^2 P + m C
-    const R scale = -dt * vol[p] * R(4) * m.inv_dx * m.inv_dx;
-    const R lj = m.lam * J * (J - R(1));
-    const R mp = mass[p];
-    R aff[9];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        R st = R(0);
-#pragma unroll
-        for (int t = 0; t < 3; ++t) st += (m.two_mu * (Fn[r * 3 + t] - Rot[r * 3 + t])) * Fn[s * 3 + t];
-        if (r == s) st += lj;
-        aff[r * 3 + s] = scale * st + mp * Cm[r * 3 + s];
-      }
-    R mom0[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) mom0[a] = mp * vp[a];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-          if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
-          const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
-          const R dp[3] = {(R(i) - fx[0]) * m.dx, (R(j) - fx[1]) * m.dx, (R(k) - fx[2]) * m.dx};
-          const R wt = w[i][0] * w[j][1] * w[k][2];
-          R* g = grid + 4 * node;
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const R mom = mom0[a] + (aff[a * 3 + 0] * dp[0] + aff[a * 3 + 1] * dp[1] + aff[a * 3 + 2] * dp[2]) +
-                          dt * pf[a];
-            atomicAdd(g + a, wt * mom);
-          }
-          atomicAdd(g + 3, wt * mp);
+          for (int t = 0; t < 3; ++t) acc += ((r == t ? R(1) : R(0)) + dt * Cm[r * 3 + t]) * Fo[t * 3 + s2];
+          Fn[r * 3 + s2] = acc;
         }
+#pragma unroll
+      for (int r = 0; r < 9; ++r) Fd[r * cap + p] = Fn[r];
+      const R J = det3(Fn);
+      R Rot[9];
+      rotation_factor<R>(Fn, Rot, J);
+      // P = 2 mu (F - R) F^T + lam J (J - 1) I; affine = -dt vol 4/dx^2 P + m C
+      const R scale = -dt * vp_vol * R(4) * m.inv_dx * m.inv_dx;
+      const R lj = m.lam * J * (J - R(1));
+      R aff[9];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int s2 = 0; s2 < 3; ++s2) {
+          R st = R(0);
+#pragma unroll
+          for (int t = 0; t < 3; ++t) st += (m.two_mu * (Fn[r * 3 + t] - Rot[r * 3 + t])) * Fn[s2 * 3 + t];
+          if (r == s2) st += lj;
+          aff[r * 3 + s2] = scale * st + mp * Cm[r * 3 + s2];
+        }
+      R mom0[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) mom0[a] = mp * vp[a] + dt * pf[a];
+      if (use_lds) {
+        // rotated walk over the 27 nodes, as for the coupling scatter above
+        int q = lane % 27;
+        for (int o = 0; o < 27; ++o) {
+          int i, j, k;
+          stencil_ijk(q, i, j, k);
+          const R dp[3] = {(R(i) - fx[0]) * m.dx, (R(j) - fx[1]) * m.dx, (R(k) - fx[2]) * m.dx};
+          const R wt = pick3(i, w[0][0], w[1][0], w[2][0]) * pick3(j, w[0][1], w[1][1], w[2][1]) *
+                       pick3(k, w[0][2], w[1][2], w[2][2]);
+          const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            atomicAdd(&FSI_TILE(ni, nj, nk, a),
+                      wt * (mom0[a] + (aff[a * 3 + 0] * dp[0] + aff[a * 3 + 1] * dp[1] + aff[a * 3 + 2] * dp[2])));
+          atomicAdd(&FSI_TILE(ni, nj, nk, 3), wt * mp);
+          q = q == 26 ? 0 : q + 1;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+              if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+              const R dp[3] = {(R(i) - fx[0]) * m.dx, (R(j) - fx[1]) * m.dx, (R(k) - fx[2]) * m.dx};
+              const R wt = w[i][0] * w[j][1] * w[k][2];
+              const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+#pragma unroll
+              for (int a = 0; a < 3; ++a)
+                atomicAdd(grid + 4 * node + a,
+                          wt * (mom0[a] + (aff[a * 3 + 0] * dp[0] + aff[a * 3 + 1] * dp[1] + aff[a * 3 + 2] * dp[2])));
+              atomicAdd(grid + 4 * node + 3, wt * mp);
+            }
+      }
+    }
+  }
+
+#undef FSI_TILE
+#undef FSI_GLOBAL
+  // flush the tile: one global atomic per non-zero entry of an in-lattice node
+  if (use_lds) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < tvals; t += blockDim.x) {
+      const R val = R(tile[t]);
+      if (val == R(0)) continue;
+      const int slot = t / tn_all;
+      const int tn = t - slot * tn_all;
+      const int lk = tn % tzi;
+      const int lj = (tn / tzi) % tyi;
+      const int li = tn / (tzi * tyi);
+      const int ni = x0 + li, nj = y0 + lj, nk = z0 + lk;
+      if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+      const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+      if (slot < 4)
+        atomicAdd(grid + 4 * node + slot, val);
+      else
+        atomicAdd(cforce + 4 * node + (slot - 4), val);
+    }
   }
 }
 
 // ------------------------------------------------- grid + G2P pass ----
 
-// fsi/mpm3d.py:268-288
+// fsi/mpm3d.py:268-288 (grid_op) for one node, evaluated where G2P reads it:
+// v = p/m + dt g on nodes with mass, then the sticky/separating walls.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_grid_op(MpmK<R> m, R* __restrict__ grid) {
-  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t N = (int64_t)m.nx * m.ny * m.nz;
-  if (node >= N) return;
-  R* g = grid + 4 * node;
+__device__ __forceinline__ void grid_velocity(const MpmK<R>& m, const R g[4], int i, int j, int k,
+                                              R out[3]) {
+  // branch-free: nodes without mass keep their (zero) momentum
   const R mass = g[3];
-  if (!(mass > R(0))) return;
-  const int k = (int)(node % m.nz);
-  const int64_t ij = node / m.nz;
-  const int j = (int)(ij % m.ny);
-  const int i = (int)(ij / m.ny);
+  const bool has = mass > R(0);
   const int idx[3] = {i, j, k};
   const int n[3] = {m.nx, m.ny, m.nz};
-  R vv[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) vv[a] = g[a] / mass + m.dt * m.g[a];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    if (idx[a] < m.bw && vv[a] < R(0)) vv[a] = -m.damping * vv[a];
-    if (idx[a] >= n[a] - m.bw && vv[a] > R(0)) vv[a] = -m.damping * vv[a];
-    g[a] = vv[a];
+    R va = g[a] / mass + m.dt * m.g[a];
+    const bool flip = (idx[a] < m.bw && va < R(0)) || (idx[a] >= n[a] - m.bw && va > R(0));
+    va = flip ? -m.damping * va : va;
+    out[a] = has ? va : g[a];
   }
 }
 
@@ -501,17 +690,19 @@ __device__ __forceinline__ bool contact_test(const MpmK<R>& m, const CplK<R>& cp
     for (int j = 0; j < 3; ++j)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-        if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
-        const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+        bool in;
+        const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
+        const R phi = cforce[4 * node + 3];
+        const int32_t sflag = solid[node];
         const R wt = w[i][0] * w[j][1] * w[k][2];
-        dyn += wt * cforce[4 * node + 3];
-        if (solid[node] != 0) stat += wt;
+        dyn += in ? wt * phi : R(0);
+        stat += (in && sflag != 0) ? wt : R(0);
       }
   return dyn >= cp.contact_thr || stat >= cp.contact_thr;
 }
 
-// fsi/mpm3d.py:290-328 (+ contact of fsi/coupling.py:463-496 when CONTACT)
+// fsi/mpm3d.py:268-328: grid update (per gathered node) + g2p + clamp, and
+// the contact pass of fsi/coupling.py:463-496 when CONTACT
 template <typename R, bool CONTACT>
 __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
                                                 R* __restrict__ v, R* __restrict__ C,
@@ -536,26 +727,45 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
     bspline(fx, w);
     R nv[3] = {R(0), R(0), R(0)};
     R nc[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+    // all 27 nodes are loaded before any is used: one memory round trip
+    R gn[27][4];
+    uint32_t inmask = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-          if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
-          const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+          bool in;
+          const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
+          const R* gp = grid + 4 * node;
+          const int q = (i * 3 + j) * 3 + k;
+#pragma unroll
+          for (int a = 0; a < 4; ++a) gn[q][a] = gp[a];
+          inmask |= (uint32_t)in << q;
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int q = (i * 3 + j) * 3 + k;
+          const bool in = (inmask >> q) & 1u;
+          const R* g4 = gn[q];
+          R gv[3];
+          grid_velocity(m, g4, base[0] + i, base[1] + j, base[2] + k, gv);
           const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
           const R wt = w[i][0] * w[j][1] * w[k][2];
-          const R* g = grid + 4 * node;
-          const R gv[3] = {g[0], g[1], g[2]};
+          // out-of-lattice nodes contribute exact zeros through selects (no
+          // branch, so all 27 nodes' loads are in flight together)
 #pragma unroll
-          for (int a = 0; a < 3; ++a) nv[a] += wt * gv[a];
+          for (int a = 0; a < 3; ++a) nv[a] += in ? wt * gv[a] : R(0);
           const R s = m.four_inv_dx * wt;
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
+            for (int c = 0; c < 3; ++c) nc[r * 3 + c] += in ? s * (gv[r] * dp[c]) : R(0);
         }
 #pragma unroll
     for (int r = 0; r < 9; ++r) C[r * cap + p] = m.apic ? nc[r] : R(0);
@@ -637,11 +847,12 @@ int check_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
 
 template <typename R>
 int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
-          const fsi_coupling_buffers* cb, int clear_force, int do_p2g, hipStream_t st) {
+          const fsi_coupling_buffers* cb, int clear_force, int do_p2g, bool zero_counters,
+          hipStream_t st) {
   MpmK<R> m = make_mpm<R>(p);
   CplK<R> cp = make_cpl<R>(c);
   const bool couple = c && c->enabled;
-  if (couple) {
+  if (couple && zero_counters) {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 0, 5);
     FSI_TRY_LAUNCH("zero_counters");
   }
@@ -668,14 +879,11 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
 
 template <typename R>
 int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
-               const fsi_coupling_buffers* cb, hipStream_t st) {
+               const fsi_coupling_buffers* cb, bool zero_counters, hipStream_t st) {
   MpmK<R> m = make_mpm<R>(p);
   CplK<R> cp = make_cpl<R>(c);
-  const int64_t N = (int64_t)m.nx * m.ny * m.nz;
-  hipLaunchKernelGGL(k_grid_op<R>, dim3(blocks_for(N)), dim3(kBlock), 0, st, m, (R*)b->grid);
-  FSI_TRY_LAUNCH("mpm_grid_op");
   const bool contact = c && c->contact_enabled;
-  if (contact) {
+  if (contact && zero_counters) {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 3, 2);
     FSI_TRY_LAUNCH("zero_counters");
   }
@@ -690,6 +898,14 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
                        (const int32_t*)nullptr, (const R*)nullptr, (int32_t*)nullptr,
                        (int32_t*)nullptr);
   FSI_TRY_LAUNCH("mpm_g2p");
+  return 0;
+}
+
+template <typename R>
+int clear_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, int32_t* counters, hipStream_t st) {
+  const int64_t n = 4LL * p->nx * p->ny * p->nz;
+  hipLaunchKernelGGL(k_clear<R>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, (R*)b->grid, counters);
+  FSI_TRY_LAUNCH("mpm_clear_grid");
   return 0;
 }
 
@@ -718,14 +934,9 @@ extern "C" {
 int fsi_mpm_clear_grid(const fsi_mpm_params* p, const fsi_mpm_buffers* b, void* stream) {
   if (int rc = check_mpm(p, b)) return rc;
   FSI_REQUIRE(b->grid, "mpm_clear_grid: null grid");
-  const int64_t n = 4LL * p->nx * p->ny * p->nz;
   hipStream_t st = (hipStream_t)stream;
-  if (p->real == FSI_REAL_F64)
-    hipLaunchKernelGGL(k_clear<double>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, (double*)b->grid);
-  else
-    hipLaunchKernelGGL(k_clear<float>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, (float*)b->grid);
-  FSI_TRY_LAUNCH("mpm_clear_grid");
-  return 0;
+  if (p->real == FSI_REAL_F64) return clear_t<double>(p, b, nullptr, st);
+  return clear_t<float>(p, b, nullptr, st);
 }
 
 int fsi_mpm_p2g(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
@@ -739,8 +950,8 @@ int fsi_mpm_p2g(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_cou
                     cb->coupling_mask && cb->counters,
                 "mpm_p2g: coupling needs all coupling buffers");
   hipStream_t st = (hipStream_t)stream;
-  if (p->real == FSI_REAL_F64) return p2g_t<double>(p, b, c, cb, clear_force, do_p2g, st);
-  return p2g_t<float>(p, b, c, cb, clear_force, do_p2g, st);
+  if (p->real == FSI_REAL_F64) return p2g_t<double>(p, b, c, cb, clear_force, do_p2g, true, st);
+  return p2g_t<float>(p, b, c, cb, clear_force, do_p2g, true, st);
 }
 
 int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
@@ -751,8 +962,8 @@ int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
     FSI_REQUIRE(cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters,
                 "mpm_grid_g2p: contact needs the coupling buffers");
   hipStream_t st = (hipStream_t)stream;
-  if (p->real == FSI_REAL_F64) return grid_g2p_t<double>(p, b, c, cb, st);
-  return grid_g2p_t<float>(p, b, c, cb, st);
+  if (p->real == FSI_REAL_F64) return grid_g2p_t<double>(p, b, c, cb, true, st);
+  return grid_g2p_t<float>(p, b, c, cb, true, st);
 }
 
 int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
@@ -769,14 +980,36 @@ int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_cou
 int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
                      const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
                      int32_t substeps, int32_t build_first, int32_t clear_force, void* stream) {
+  if (int rc = check_mpm(p, b)) return rc;
   FSI_REQUIRE(substeps >= 0, "mpm_substeps: negative substep count");
+  FSI_REQUIRE(b->x && b->v && b->C && b->F && b->mass && b->volume && b->active && b->pforce &&
+                  b->grid,
+              "mpm_substeps: null particle or grid buffer");
+  const bool counters = c && (c->enabled || c->contact_enabled);
+  if (c && c->enabled)
+    FSI_REQUIRE(cb && cb->solid && cb->lbm_vel && cb->cforce && cb->valid_weight &&
+                    cb->coupling_mask && cb->counters,
+                "mpm_substeps: coupling needs all coupling buffers");
+  if (c && c->contact_enabled)
+    FSI_REQUIRE(cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters,
+                "mpm_substeps: contact needs the coupling buffers");
+  hipStream_t st = (hipStream_t)stream;
   fsi_coupling_params local{};
   if (c) local = *c;
+  const bool f64 = p->real == FSI_REAL_F64;
+  int32_t* ctr = counters ? cb->counters : nullptr;
+  // three launches per substep: clear (+ counters), P2G (+ coupling),
+  // grid update + G2P (+ contact)
   for (int s = 0; s < substeps; ++s) {
     local.build_fraction = (s == 0 && build_first) ? 1 : 0;
-    if (int rc = fsi_mpm_clear_grid(p, b, stream)) return rc;
-    if (int rc = fsi_mpm_p2g(p, b, c ? &local : nullptr, cb, clear_force, 1, stream)) return rc;
-    if (int rc = fsi_mpm_grid_g2p(p, b, c ? &local : nullptr, cb, stream)) return rc;
+    const fsi_coupling_params* lc = c ? &local : nullptr;
+    int rc = f64 ? clear_t<double>(p, b, ctr, st) : clear_t<float>(p, b, ctr, st);
+    if (rc) return rc;
+    rc = f64 ? p2g_t<double>(p, b, lc, cb, clear_force, 1, false, st)
+             : p2g_t<float>(p, b, lc, cb, clear_force, 1, false, st);
+    if (rc) return rc;
+    rc = f64 ? grid_g2p_t<double>(p, b, lc, cb, false, st) : grid_g2p_t<float>(p, b, lc, cb, false, st);
+    if (rc) return rc;
   }
   return 0;
 }
