@@ -153,6 +153,10 @@ typedef struct fsi_mpm_buffers {
   const int32_t* active;   /* cap */
   void* pforce;            /* 3*cap real */
   void* grid;              /* 4N real, AoS (momentum/velocity xyz, mass) per node */
+  void* grid_alt;          /* optional second 4N real grid (may be NULL): with it,
+                              fsi_mpm_substeps alternates the two grids and each
+                              G2P pass zeroes the one the next P2G fills, so a
+                              substep needs no separate clear launch */
 } fsi_mpm_buffers;
 
 /* Coupling inputs/outputs read and written inside the particle passes

@@ -329,8 +329,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
                                                 const R* __restrict__ lbm_vel, R* __restrict__ cforce,
                                                 R* __restrict__ valid_weight,
                                                 int32_t* __restrict__ cmask,
-                                                int32_t* __restrict__ counters, int clear_force) {
+                                                int32_t* __restrict__ counters, int clear_force,
+                                                int32_t* __restrict__ zero_ctr) {
   __shared__ Tile::acc tile[Tile::kNodes * Tile::kValues];
+  // batched substeps: the G2P-pass counters of this substep start at zero
+  if (zero_ctr && blockIdx.x == 0 && threadIdx.x < 2) zero_ctr[threadIdx.x] = 0;
   __shared__ int red[6][kBlock / 64];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = p < m.num;
@@ -711,8 +714,14 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
                                                 const int32_t* __restrict__ solid,
                                                 const R* __restrict__ cforce,
                                                 int32_t* __restrict__ contact_mask,
-                                                int32_t* __restrict__ counters) {
+                                                int32_t* __restrict__ counters,
+                                                uint4* __restrict__ zero_grid, int64_t zero_n16,
+                                                int32_t* __restrict__ zero_ctr) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  // batched substeps: zero the other grid (read by nobody in this pass) for
+  // the next P2G pass, and that pass's counters
+  for (int64_t t = p; t < zero_n16; t += (int64_t)gridDim.x * blockDim.x) zero_grid[t] = make_uint4(0, 0, 0, 0);
+  if (zero_ctr && p < 3) zero_ctr[p] = 0;
   const bool live = p < m.num;
   const bool act = live && active[p] != 0;
   const int cap = m.cap;
@@ -848,7 +857,7 @@ int check_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
 template <typename R>
 int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
           const fsi_coupling_buffers* cb, int clear_force, int do_p2g, bool zero_counters,
-          hipStream_t st) {
+          R* grid_target, int32_t* zero_ctr, hipStream_t st) {
   MpmK<R> m = make_mpm<R>(p);
   CplK<R> cp = make_cpl<R>(c);
   const bool couple = c && c->enabled;
@@ -860,10 +869,10 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
   const dim3 grid(blocks_for(m.num)), block(kBlock);
 #define FSI_P2G_ARGS                                                                            \
   m, cp, (R*)b->x, (R*)b->v, (const R*)b->C, (R*)b->F, (const R*)b->mass, (const R*)b->volume, \
-      b->active, (R*)b->pforce, (R*)b->grid, cb ? cb->solid : nullptr,                          \
+      b->active, (R*)b->pforce, grid_target, cb ? cb->solid : nullptr,                          \
       cb ? (const R*)cb->lbm_vel : nullptr, cb ? (R*)cb->cforce : nullptr,                      \
       cb ? (R*)cb->valid_weight : nullptr, cb ? cb->coupling_mask : nullptr,                    \
-      cb ? cb->counters : nullptr, clear_force
+      cb ? cb->counters : nullptr, clear_force, zero_ctr
   if (couple && do_p2g)
     hipLaunchKernelGGL((k_p2g<R, true, true>), grid, block, 0, st, FSI_P2G_ARGS);
   else if (couple)
@@ -879,7 +888,8 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
 
 template <typename R>
 int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
-               const fsi_coupling_buffers* cb, bool zero_counters, hipStream_t st) {
+               const fsi_coupling_buffers* cb, bool zero_counters, const R* grid_src,
+               void* zero_grid, int32_t* zero_ctr, hipStream_t st) {
   MpmK<R> m = make_mpm<R>(p);
   CplK<R> cp = make_cpl<R>(c);
   const bool contact = c && c->contact_enabled;
@@ -887,25 +897,67 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 3, 2);
     FSI_TRY_LAUNCH("zero_counters");
   }
-  if (m.num == 0) return 0;
+  // the zeroing of the other grid (16-byte words) is spread over at most
+  // 1024 workgroups; particle workgroups beyond that also take part
+  const int64_t n16 = zero_grid ? (4LL * p->nx * p->ny * p->nz * (int64_t)sizeof(R)) / 16 : 0;
+  const int64_t zero_blocks = n16 ? std::min<int64_t>(blocks_for(n16), 1024) : 0;
+  const int64_t blocks = std::max<int64_t>(m.num ? blocks_for(m.num) : 0, std::max<int64_t>(zero_blocks, zero_ctr ? 1 : 0));
+  if (blocks == 0) return 0;
   if (contact)
-    hipLaunchKernelGGL((k_g2p<R, true>), dim3(blocks_for(m.num)), dim3(kBlock), 0, st, m, cp,
-                       (R*)b->x, (R*)b->v, (R*)b->C, b->active, (const R*)b->grid, cb->solid,
-                       (const R*)cb->cforce, cb->contact_mask, cb->counters);
+    hipLaunchKernelGGL((k_g2p<R, true>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, cp,
+                       (R*)b->x, (R*)b->v, (R*)b->C, b->active, grid_src, cb->solid,
+                       (const R*)cb->cforce, cb->contact_mask, cb->counters, (uint4*)zero_grid, n16,
+                       zero_ctr);
   else
-    hipLaunchKernelGGL((k_g2p<R, false>), dim3(blocks_for(m.num)), dim3(kBlock), 0, st, m, cp,
-                       (R*)b->x, (R*)b->v, (R*)b->C, b->active, (const R*)b->grid,
+    hipLaunchKernelGGL((k_g2p<R, false>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, cp,
+                       (R*)b->x, (R*)b->v, (R*)b->C, b->active, grid_src,
                        (const int32_t*)nullptr, (const R*)nullptr, (int32_t*)nullptr,
-                       (int32_t*)nullptr);
+                       (int32_t*)nullptr, (uint4*)zero_grid, n16, zero_ctr);
   FSI_TRY_LAUNCH("mpm_g2p");
   return 0;
 }
+
+template <typename R>
+int substeps_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+               const fsi_coupling_buffers* cb, int32_t* ctr, int substeps, int build_first,
+               int clear_force, hipStream_t st);
 
 template <typename R>
 int clear_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, int32_t* counters, hipStream_t st) {
   const int64_t n = 4LL * p->nx * p->ny * p->nz;
   hipLaunchKernelGGL(k_clear<R>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, (R*)b->grid, counters);
   FSI_TRY_LAUNCH("mpm_clear_grid");
+  return 0;
+}
+
+// fsi/mpm3d.py:189-214 repeated `substeps` times. With one grid: clear
+// (+ counters), P2G (+ coupling), grid update + G2P (+ contact) per substep.
+// With a second grid the clear runs once: substep s scatters into grid s%2,
+// and its G2P pass zeroes grid (s+1)%2 and counters 0..2 for substep s+1
+// while P2G zeroes counters 3..4, so a substep is two launches.
+template <typename R>
+int substeps_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+               const fsi_coupling_buffers* cb, int32_t* ctr, int substeps, int build_first,
+               int clear_force, hipStream_t st) {
+  fsi_coupling_params local{};
+  if (c) local = *c;
+  R* grids[2] = {(R*)b->grid, (R*)b->grid_alt};
+  const bool two = b->grid_alt != nullptr && b->grid_alt != b->grid;
+  if (two && substeps > 0)
+    if (int rc = clear_t<R>(p, b, ctr, st)) return rc;
+  for (int s = 0; s < substeps; ++s) {
+    local.build_fraction = (s == 0 && build_first) ? 1 : 0;
+    const fsi_coupling_params* lc = c ? &local : nullptr;
+    R* g = two ? grids[s & 1] : grids[0];
+    if (!two)
+      if (int rc = clear_t<R>(p, b, ctr, st)) return rc;
+    int32_t* zc_p2g = (two && ctr && s > 0) ? ctr + 3 : nullptr;
+    if (int rc = p2g_t<R>(p, b, lc, cb, clear_force, 1, false, g, zc_p2g, st)) return rc;
+    const bool more = s + 1 < substeps;
+    void* zg = (two && more) ? (void*)grids[(s + 1) & 1] : nullptr;
+    int32_t* zc_g2p = (two && more && ctr) ? ctr : nullptr;
+    if (int rc = grid_g2p_t<R>(p, b, lc, cb, false, g, zg, zc_g2p, st)) return rc;
+  }
   return 0;
 }
 
@@ -950,8 +1002,9 @@ int fsi_mpm_p2g(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_cou
                     cb->coupling_mask && cb->counters,
                 "mpm_p2g: coupling needs all coupling buffers");
   hipStream_t st = (hipStream_t)stream;
-  if (p->real == FSI_REAL_F64) return p2g_t<double>(p, b, c, cb, clear_force, do_p2g, true, st);
-  return p2g_t<float>(p, b, c, cb, clear_force, do_p2g, true, st);
+  if (p->real == FSI_REAL_F64)
+    return p2g_t<double>(p, b, c, cb, clear_force, do_p2g, true, (double*)b->grid, nullptr, st);
+  return p2g_t<float>(p, b, c, cb, clear_force, do_p2g, true, (float*)b->grid, nullptr, st);
 }
 
 int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
@@ -962,8 +1015,9 @@ int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
     FSI_REQUIRE(cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters,
                 "mpm_grid_g2p: contact needs the coupling buffers");
   hipStream_t st = (hipStream_t)stream;
-  if (p->real == FSI_REAL_F64) return grid_g2p_t<double>(p, b, c, cb, true, st);
-  return grid_g2p_t<float>(p, b, c, cb, true, st);
+  if (p->real == FSI_REAL_F64)
+    return grid_g2p_t<double>(p, b, c, cb, true, (const double*)b->grid, nullptr, nullptr, st);
+  return grid_g2p_t<float>(p, b, c, cb, true, (const float*)b->grid, nullptr, nullptr, st);
 }
 
 int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
@@ -998,20 +1052,8 @@ int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
   if (c) local = *c;
   const bool f64 = p->real == FSI_REAL_F64;
   int32_t* ctr = counters ? cb->counters : nullptr;
-  // three launches per substep: clear (+ counters), P2G (+ coupling),
-  // grid update + G2P (+ contact)
-  for (int s = 0; s < substeps; ++s) {
-    local.build_fraction = (s == 0 && build_first) ? 1 : 0;
-    const fsi_coupling_params* lc = c ? &local : nullptr;
-    int rc = f64 ? clear_t<double>(p, b, ctr, st) : clear_t<float>(p, b, ctr, st);
-    if (rc) return rc;
-    rc = f64 ? p2g_t<double>(p, b, lc, cb, clear_force, 1, false, st)
-             : p2g_t<float>(p, b, lc, cb, clear_force, 1, false, st);
-    if (rc) return rc;
-    rc = f64 ? grid_g2p_t<double>(p, b, lc, cb, false, st) : grid_g2p_t<float>(p, b, lc, cb, false, st);
-    if (rc) return rc;
-  }
-  return 0;
+  if (f64) return substeps_t<double>(p, b, c, cb, ctr, substeps, build_first, clear_force, st);
+  return substeps_t<float>(p, b, c, cb, ctr, substeps, build_first, clear_force, st);
 }
 
 int fsi_polar_rotation(int32_t real, int32_t n, const void* F, void* R, void* stream) {

@@ -64,7 +64,7 @@ class MpmParams(ctypes.Structure):
 
 class MpmBuffers(ctypes.Structure):
     _fields_ = [("x", _vp), ("v", _vp), ("C", _vp), ("F", _vp), ("mass", _vp), ("volume", _vp),
-                ("active", _vp), ("pforce", _vp), ("grid", _vp)]
+                ("active", _vp), ("pforce", _vp), ("grid", _vp), ("grid_alt", _vp)]
 
 
 class CouplingParams(ctypes.Structure):

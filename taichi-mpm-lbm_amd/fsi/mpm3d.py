@@ -61,11 +61,14 @@ class MPMSolver3D:
         self.active = torch.zeros(cap, dtype=torch.int32, device=dev)
         self.particle_force = torch.zeros(3 * cap, dtype=rd, device=dev)
         self.grid = torch.zeros(4 * self.nx * self.ny * self.nz, dtype=rd, device=dev)
+        # second grid: batched substeps alternate the two, each G2P pass
+        # zeroing the one the next P2G pass fills
+        self.grid_alt = torch.zeros_like(self.grid)
         self.num_particles = 0
         self._buffers = nat.MpmBuffers(
             nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
             nat.ptr(self.mass), nat.ptr(self.volume), nat.ptr(self.active),
-            nat.ptr(self.particle_force), nat.ptr(self.grid),
+            nat.ptr(self.particle_force), nat.ptr(self.grid), nat.ptr(self.grid_alt),
         )
         self._initialized = False
 
