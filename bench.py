@@ -81,7 +81,7 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(mask, budget_s=12.0, max_steps=3):
+def cpu_baseline(mask, budget_s=12.0, max_steps=40):
     """The CPU oracle (C++ restatement in the reference's operation order,
     FP32, OpenMP over the host cores) on the same 256^3 scene; bounded sample."""
     import numpy as np
@@ -156,6 +156,22 @@ def time_coupled(steps, warmup):
             "particles": int(len(pos)), "substeps": 112, "timed_steps": steps}
 
 
+def calibrate_copy(n):
+    """A plain device-to-device copy of one FP32 population buffer
+    (19 * n^3 * 4 bytes), three times: the known-size kernel that scales the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import torch
+
+    nbytes = 19 * n ** 3 * 4
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,11 +183,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-coupled", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of one population buffer (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding "
               "of the lattice is a design note, not part of this build.", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy(args.size)
+        return
 
     import numpy as np
     import torch

@@ -208,7 +208,10 @@ int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
 /* The substep loop of LBMMpmCoupler.step (fsi/coupling.py:147-160) and of
  * repeated MPMSolver3D.substep calls: `substeps` x (clear grid, P2G with the
  * optional fused coupling, grid update + G2P with optional contact), one call.
- * The occupancy phi is scattered on the first substep only when build_first. */
+ * The occupancy phi is scattered on the first substep only when build_first.
+ * With b->grid_alt set the grid is cleared once per call and each substep is
+ * two launches (the G2P pass zeroes the other grid for the next P2G pass);
+ * the last substep's P2G result is then in grid or grid_alt by parity. */
 int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
                      const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
                      int32_t substeps, int32_t build_first, int32_t clear_force, void* stream);
