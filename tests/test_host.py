@@ -5,6 +5,7 @@ before any device work, the import surface, and the C ABI symbol table."""
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from fsi.config import (
@@ -130,3 +131,14 @@ def test_c_abi_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert lib.fsi_abi_version() == 1
     assert set(names) >= set(_native._SIGNATURES)
+
+
+def test_lbm_load_solid_from_txt(tmp_path):
+    # fsi/lbm3d.py load_solid_from_txt: whitespace text, Fortran-order reshape
+    from fsi.lbm3d import LBMSolver3D
+
+    solid = (np.arange(4 * 3 * 2) % 3 == 0).astype(np.int32).reshape(4, 3, 2)
+    path = tmp_path / "mask.txt"
+    np.savetxt(path, solid.reshape(-1, order="F"), fmt="%d")
+    got = LBMSolver3D.load_solid_from_txt(str(path), 4, 3, 2)
+    assert np.array_equal(got, solid)
