@@ -243,6 +243,32 @@ int fsi_force_assemble(const fsi_lbm_params* lp, const fsi_lbm_buffers* lb,
                        const fsi_force_assembly* a, void* cforce, void* ib_force, void* ib_total,
                        int32_t* counters, int32_t write_force, void* stream);
 
+/* ------------------------------------------------------- reductions ---- */
+
+/* doubles per workgroup row of a reduction workspace */
+#define FSI_SUM_SLOTS 16
+
+/* FP64 sums behind LBMSolver3D.total_mass (fsi/lbm3d.py:449-452),
+ * max_velocity_norm (454-456) and LBMMpmCoupler.total_fluid_coupling_force
+ * (fsi/coupling.py:241-244), in one pass: out[0] = sum of rho over fluid
+ * cells, out[1] = max |u|^2 over all cells, out[2..4] = coupling force summed
+ * over fluid cells (cforce: the 4N real AoS coupling buffer, or null for 0).
+ * workspace: device doubles, at least FSI_SUM_SLOTS (up to 1024 rows of
+ * FSI_SUM_SLOTS are used). Fixed grid and summation order: the result is
+ * reproducible bit for bit. out: 5 device doubles. */
+int fsi_reduce_lattice(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* cforce,
+                       void* workspace, int64_t workspace_doubles, double* out, void* stream);
+
+/* FP64 sums behind MPMSolver3D.center_of_mass (fsi/mpm3d.py:353-362),
+ * max_velocity_norm (364-368), LBMMpmCoupler.total_particle_coupling_force
+ * (fsi/coupling.py:238-239) and the valid-weight statistics of
+ * coupling_diagnostics (263-270), over the first num_particles particles:
+ * out[0] sum m, out[1..3] sum m x, out[4] max |v|^2, out[5..7] particle force
+ * summed, out[8] min and out[9] sum of valid_weight (cap real, or null).
+ * out: 10 device doubles. Workspace as for fsi_reduce_lattice. */
+int fsi_reduce_particles(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const void* valid_weight,
+                         void* workspace, int64_t workspace_doubles, double* out, void* stream);
+
 /* ----------------------------------------------------------- common ---- */
 
 int fsi_abi_version(void);

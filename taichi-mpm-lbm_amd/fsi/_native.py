@@ -110,6 +110,8 @@ _SIGNATURES = {
     "fsi_contact": [_vp, _vp, _vp, _vp, _vp],
     "fsi_polar_rotation": [ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp],
     "fsi_force_assemble": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp],
+    "fsi_reduce_lattice": [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
+    "fsi_reduce_particles": [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
     "fsi_abi_version": [],
     "fsi_fill_zero": [ctypes.c_int64, _vp, _vp],
 }
@@ -169,6 +171,18 @@ def call(name: str, *args) -> None:
     if rc != 0:
         msg = lib.fsi_last_error().decode(errors="replace")
         raise RuntimeError(f"{name} failed ({rc}): {msg}")
+
+
+# doubles per workgroup row of a reduction workspace (FSI_SUM_SLOTS) and the
+# workspace size the solvers allocate: 1024 rows
+SUM_SLOTS = 16
+SUM_WORKSPACE = 1024 * SUM_SLOTS
+
+
+def reduction_buffers(dev: torch.device) -> tuple[torch.Tensor, torch.Tensor]:
+    """(workspace, out) of the FP64 reductions: 128 KiB and 16 doubles."""
+    return (torch.empty(SUM_WORKSPACE, dtype=torch.float64, device=dev),
+            torch.zeros(SUM_SLOTS, dtype=torch.float64, device=dev))
 
 
 def fill_zero(t: torch.Tensor) -> None:

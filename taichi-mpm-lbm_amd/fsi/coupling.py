@@ -235,10 +235,29 @@ class LBMMpmCoupler:
     def particle_contact_mask_numpy(self) -> np.ndarray:
         return self.contact_mask[: self.mpm.particle_count()].cpu().numpy().copy()
 
+    # The accessors keep the reference's float32 host reductions;
+    # diagnostic_vector() gathers the FP64 device reductions the per-step
+    # diagnostics use (one small host copy instead of whole fields).
+
+    def _real(self):
+        return np.float64 if self.lbm.real_dtype == torch.float64 else np.float32
+
+    def diagnostic_vector(self) -> np.ndarray:
+        """One host copy of every reduced quantity, float64:
+        [0:5] lattice sums (fluid mass, max |u|^2, fluid coupling force xyz),
+        [5:15] particle sums (sum m, sum m x, max |v|^2, particle force xyz,
+        min and sum of valid weights), [15:23] counters, [23:26] IB total."""
+        lat = self.lbm.lattice_sums(self.cforce)
+        part = self.mpm.particle_sums(self.valid_weight)
+        return torch.cat([lat, part, self.counters.to(torch.float64),
+                          self.ib_total.to(torch.float64)]).cpu().numpy()
+
     def total_particle_coupling_force(self) -> np.ndarray:
+        # fsi/coupling.py:238-239
         return self.mpm.particle_forces_numpy().sum(axis=0)
 
     def total_fluid_coupling_force(self) -> np.ndarray:
+        # fsi/coupling.py:241-244
         force_np = self.coupling_force_numpy()
         solid_np = self.lbm.solid_numpy()
         return force_np[solid_np == 0].sum(axis=0)
@@ -246,9 +265,9 @@ class LBMMpmCoupler:
     def _counters(self) -> np.ndarray:
         return self.counters.cpu().numpy()
 
-    def immersed_boundary_diagnostics(self) -> dict[str, int | float]:
-        ctr = self._counters()
-        ib_total_force = self.ib_total.cpu().numpy().astype(np.float64)
+    @staticmethod
+    def _ib_dict(ctr, ib_total_force) -> dict[str, int | float]:
+        ib_total_force = np.asarray(ib_total_force, dtype=np.float64)
         return {
             "ib_active_cell_count": int(ctr[_IB_ACTIVE]),
             "ib_clipped_cell_count": int(ctr[_IB_CLIPPED]),
@@ -260,15 +279,26 @@ class LBMMpmCoupler:
             "contact_damped_particle_count": int(ctr[_DAMPED]),
         }
 
-    def coupling_diagnostics(self) -> dict[str, int | float]:
-        valid_weights = self.particle_valid_weights_numpy()
-        if len(valid_weights) == 0:
-            min_weight = 0.0
-            mean_weight = 0.0
+    def immersed_boundary_diagnostics(self) -> dict[str, int | float]:
+        # fsi/coupling.py:246-257
+        return self._ib_dict(self._counters(), self.ib_total.cpu().numpy())
+
+    def coupling_diagnostics(self, vector: np.ndarray | None = None) -> dict[str, int | float]:
+        # fsi/coupling.py:263-276. Without `vector` the weight statistics are
+        # the reference's float32 host ones; with a diagnostic_vector()
+        # already on the host they come from its FP64 device sums.
+        count = self.mpm.particle_count()
+        if vector is None:
+            weights = self.particle_valid_weights_numpy()
+            ctr = self._counters()
+            ib = self.ib_total.cpu().numpy()
+            min_weight = float(weights.min()) if count else 0.0
+            mean_weight = float(weights.mean()) if count else 0.0
         else:
-            min_weight = float(valid_weights.min())
-            mean_weight = float(valid_weights.mean())
-        ctr = self._counters()
+            ctr = vector[15:23].astype(np.int64)
+            ib = vector[23:26]
+            min_weight = float(self._real()(vector[13])) if count else 0.0
+            mean_weight = float(self._real()(vector[14] / count)) if count else 0.0
         return {
             "unsupported_particle_count": int(ctr[_UNSUPPORTED]),
             "partial_support_particle_count": int(ctr[_PARTIAL]),
@@ -276,5 +306,5 @@ class LBMMpmCoupler:
             "min_particle_valid_weight": min_weight,
             "mean_particle_valid_weight": mean_weight,
             "effective_gamma": self.effective_gamma(),
-            **self.immersed_boundary_diagnostics(),
+            **self._ib_dict(ctr, ib),
         }

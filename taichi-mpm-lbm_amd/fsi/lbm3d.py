@@ -96,6 +96,7 @@ class LBMSolver3D:
         self.nbmask = torch.zeros(n, dtype=torch.int32, device=dev)
         # two slots (previous / next step) x two faces x 3 components
         self.face_vel = torch.zeros(12 * self.ny * self.nz, dtype=rd, device=dev)
+        self._sum_ws, self._sum_out = nat.reduction_buffers(dev)
 
         self._params = self._make_params()
         self._buffers = nat.LbmBuffers(
@@ -328,12 +329,27 @@ class LBMSolver3D:
     def force_numpy(self) -> np.ndarray:
         return self._aos(self.force, 3)
 
+    def lattice_sums(self, cforce: torch.Tensor | None = None) -> torch.Tensor:
+        """FP64 sums over the materialised state, on the device (5 doubles):
+        fluid mass, max |u|^2, and the coupling buffer's force summed over
+        fluid cells (0 without one). No host copy."""
+        self._ensure_macro()
+        nat.call("fsi_reduce_lattice", nat.ref(self._params), nat.ref(self._buffers), nat.ptr(cforce),
+                 nat.ptr(self._sum_ws), self._sum_ws.numel(), nat.ptr(self._sum_out))
+        return self._sum_out[:5]
+
+    # The accessors below keep the reference's float32 host reductions (its
+    # validation values depend on their rounding); the per-step diagnostics
+    # use lattice_sums().
+
     def total_mass(self) -> float:
+        # fsi/lbm3d.py:449-452
         rho_np = self.density_numpy()
         solid_np = self.solid_numpy()
         return float(rho_np[solid_np == 0].sum())
 
     def max_velocity_norm(self) -> float:
+        # fsi/lbm3d.py:454-456
         velocity_np = self.velocity_numpy()
         return float(np.linalg.norm(velocity_np, axis=-1).max())
 

@@ -64,6 +64,7 @@ class MPMSolver3D:
         # second grid: batched substeps alternate the two, each G2P pass
         # zeroing the one the next P2G pass fills
         self.grid_alt = torch.zeros_like(self.grid)
+        self._sum_ws, self._sum_out = nat.reduction_buffers(dev)
         self.num_particles = 0
         self._buffers = nat.MpmBuffers(
             nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
@@ -259,6 +260,20 @@ class MPMSolver3D:
     def masses_numpy(self) -> np.ndarray:
         return self.mass[: self.num_particles].cpu().numpy().copy()
 
+    def particle_sums(self, valid_weight: torch.Tensor | None = None) -> torch.Tensor:
+        """FP64 sums over the first particle_count() particles, on the device
+        (10 doubles): sum m, sum m x (3), max |v|^2, particle force summed
+        (3), min and sum of `valid_weight` (when given). No host copy."""
+        p = self.params(self.dt_default)
+        nat.call("fsi_reduce_particles", nat.ref(p), nat.ref(self._buffers), nat.ptr(valid_weight),
+                 nat.ptr(self._sum_ws), self._sum_ws.numel(), nat.ptr(self._sum_out))
+        return self._sum_out[:10]
+
+    # center_of_mass / max_velocity_norm keep the reference's float32 host
+    # reductions (its validation values depend on their rounding: the
+    # gravity case's centre-of-mass drop is float32-summation dominated);
+    # the per-step diagnostics use particle_sums().
+
     def center_of_mass(self) -> np.ndarray:
         # fsi/mpm3d.py:353-362
         count = self.particle_count()
@@ -271,7 +286,14 @@ class MPMSolver3D:
             return np.zeros(3, dtype=self._np_real)
         return (positions * masses[:, None]).sum(axis=0) / total_mass
 
+    def _com_from(self, sums: np.ndarray) -> np.ndarray:
+        """Centre of mass from particle_sums() copied to the host."""
+        if not sums[0] > 0.0:
+            return np.zeros(3, dtype=self._np_real)
+        return (sums[1:4] / sums[0]).astype(self._np_real)
+
     def max_velocity_norm(self) -> float:
+        # fsi/mpm3d.py:364-368
         velocities = self.velocities_numpy()
         if len(velocities) == 0:
             return 0.0

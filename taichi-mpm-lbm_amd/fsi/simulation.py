@@ -116,32 +116,34 @@ class FSISimulation:
         return self.output_writer.write_snapshot(self)
 
     def diagnostics(self) -> dict[str, object]:
-        # fsi/simulation.py:117-147, same keys and host reductions
+        # fsi/simulation.py:117-147, same keys and value types; every sum is
+        # an FP64 reduction on the device and the whole dict costs one small
+        # host copy (the reference copies the lattice and particle fields)
         self._validate_initialized()
-        coupling_diagnostics = self.coupler.coupling_diagnostics()
+        d = self.coupler.diagnostic_vector()
+        lat, part = d[0:5], d[5:15]
+        count = self.mpm.particle_count()
+        real = self.coupler._real()
+        cd = self.coupler.coupling_diagnostics(d)
         return {
             "step": self.step_index,
             "time": self.time,
-            "lbm_total_mass": self.lbm.total_mass(),
-            "lbm_max_velocity_norm": self.lbm.max_velocity_norm(),
-            "mpm_particle_count": self.mpm.particle_count(),
-            "mpm_center_of_mass": self.mpm.center_of_mass(),
-            "mpm_max_velocity_norm": self.mpm.max_velocity_norm(),
-            "total_particle_coupling_force": self.coupler.total_particle_coupling_force(),
-            "total_fluid_coupling_force": self.coupler.total_fluid_coupling_force(),
-            "coupling_unsupported_particle_count": coupling_diagnostics["unsupported_particle_count"],
-            "coupling_partial_support_particle_count": coupling_diagnostics[
-                "partial_support_particle_count"
-            ],
-            "coupling_clipped_particle_count": coupling_diagnostics["clipped_particle_count"],
-            "coupling_effective_gamma": coupling_diagnostics["effective_gamma"],
-            "coupling_ib_active_cell_count": coupling_diagnostics["ib_active_cell_count"],
-            "coupling_ib_clipped_cell_count": coupling_diagnostics["ib_clipped_cell_count"],
-            "coupling_ib_total_force_norm": coupling_diagnostics["ib_total_force_norm"],
-            "coupling_contact_candidate_count": coupling_diagnostics["contact_candidate_count"],
-            "coupling_contact_damped_particle_count": coupling_diagnostics[
-                "contact_damped_particle_count"
-            ],
+            "lbm_total_mass": float(lat[0]),
+            "lbm_max_velocity_norm": float(np.sqrt(lat[1])),
+            "mpm_particle_count": count,
+            "mpm_center_of_mass": self.mpm._com_from(part) if count else np.zeros(3, dtype=real),
+            "mpm_max_velocity_norm": float(np.sqrt(part[4])) if count else 0.0,
+            "total_particle_coupling_force": part[5:8].astype(real),
+            "total_fluid_coupling_force": lat[2:5].astype(real),
+            "coupling_unsupported_particle_count": cd["unsupported_particle_count"],
+            "coupling_partial_support_particle_count": cd["partial_support_particle_count"],
+            "coupling_clipped_particle_count": cd["clipped_particle_count"],
+            "coupling_effective_gamma": cd["effective_gamma"],
+            "coupling_ib_active_cell_count": cd["ib_active_cell_count"],
+            "coupling_ib_clipped_cell_count": cd["ib_clipped_cell_count"],
+            "coupling_ib_total_force_norm": cd["ib_total_force_norm"],
+            "coupling_contact_candidate_count": cd["contact_candidate_count"],
+            "coupling_contact_damped_particle_count": cd["contact_damped_particle_count"],
         }
 
     def _validate_initialized(self) -> None:
