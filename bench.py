@@ -135,6 +135,51 @@ def coupled_scene_32(storage="fp32"):
     return cfg, solid, pos
 
 
+def coupled_scene_large(storage="fp32"):
+    """256x128x128 coupled suspension: inflow u_x=0.04 (velocity face x-),
+    pressure face x+, no-slip walls as a solid shell in y/z, tau=0.56;
+    16 non-overlapping elastic spheres r U[6,12] (sphere_pack seed 0),
+    8 particles per cell (spacing 0.5, jitter U(-0.125,0.125)), E=1e3,
+    nu=0.3, density 1.2, gamma=0.5, 112 substeps per lattice step."""
+    from fsi.config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
+    from fsi.scenes import channel_walls, sphere_pack
+
+    shape = (256, 128, 128)
+    pos, info = sphere_pack(shape, 16, 6.0, 12.0, spacing=0.5, seed=0)
+    lbm = LBMConfig(nx=shape[0], ny=shape[1], nz=shape[2], viscosity=0.06 / 3.0, rho0=1.0,
+                    x_left=BoundaryConfig("velocity", velocity=(0.04, 0.0, 0.0)),
+                    x_right=BoundaryConfig("pressure", rho=1.0), storage=storage)
+    mpm = MPMConfig(nx=shape[0], ny=shape[1], nz=shape[2], max_particles=len(pos), dx=1.0, dt=1.0 / 112,
+                    density=1.2, youngs_modulus=1.0e3, poisson_ratio=0.3, gravity=(0.0, 0.0, 0.0),
+                    boundary_width=2)
+    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=112)
+    cfg = SimulationConfig(num_steps=100, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
+    return cfg, channel_walls(shape), pos
+
+
+def time_coupled_large(steps, warmup):
+    import torch
+
+    import fsi
+
+    cfg, solid, pos = coupled_scene_large()
+    sim = fsi.FSISimulation(cfg)
+    sim.initialize_lbm(solid)
+    sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+    for _ in range(warmup):
+        sim.advance()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.advance()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n_sub = cfg.coupling.mpm_substeps_per_lbm_step
+    return {"steps_per_s": round(steps / dt, 3), "ms_per_step": round(1e3 * dt / steps, 3),
+            "particles": int(len(pos)), "substeps": n_sub, "timed_steps": steps,
+            "particle_substeps_per_s": round(len(pos) * n_sub * steps / dt, 1)}
+
+
 def time_coupled(steps, warmup):
     import torch
 
@@ -183,6 +228,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-coupled", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=3,
+                    help="timed steps of the 256x128x128 coupled suspension (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of one population buffer (counter calibration)")
     args = ap.parse_args()
@@ -273,6 +320,10 @@ def main():
     if not args.no_coupled:
         line["coupled"] = {"workload": "32^3 coupled penalty scene, FP32, 4096 particles, 112 substeps",
                            **time_coupled(args.coupled_steps, 2)}
+    if args.large_steps > 0:
+        line["coupled_large"] = {
+            "workload": "256x128x128 coupled suspension, FP32, 16 elastic spheres, 112 substeps",
+            **time_coupled_large(args.large_steps, 1)}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(mask)
     print(json.dumps(line), flush=True)

@@ -141,6 +141,8 @@ typedef struct fsi_mpm_params {
   double boundary_damping;
   int32_t apic;
   double clamp_lower, clamp_upper[3];
+  int32_t num_chunks;      /* entries of fsi_mpm_buffers.chunks (0: P2G takes the
+                              particles in slot order, 256 per workgroup) */
 } fsi_mpm_params;
 
 typedef struct fsi_mpm_buffers {
@@ -157,6 +159,9 @@ typedef struct fsi_mpm_buffers {
                               fsi_mpm_substeps alternates the two grids and each
                               G2P pass zeroes the one the next P2G fills, so a
                               substep needs no separate clear launch */
+  const int32_t* chunks;   /* optional P2G work list from fsi_mpm_sort: (first slot,
+                              count <= 256) pairs, one workgroup each; every chunk
+                              lies in one 4x4x4 cell block */
 } fsi_mpm_buffers;
 
 /* Coupling inputs/outputs read and written inside the particle passes
@@ -215,6 +220,28 @@ int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
 int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
                      const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
                      int32_t substeps, int32_t build_first, int32_t clear_force, void* stream);
+
+/* Device workspace (bytes) fsi_mpm_sort needs for this grid and capacity. */
+int fsi_mpm_sort_workspace(const fsi_mpm_params* p, int64_t* bytes);
+
+/* Cell-key counting sort of the first num_particles particles (no
+ * counterpart in the reference, whose particles keep their input order; it
+ * changes only the order in which particles are processed). Key: the
+ * truncating base cell of fsi/mpm3d.py:239 (computed as P2G computes it),
+ * clamped to [-1, n] per axis, in 4x4x4-blocked order; inactive particles
+ * last. The permutation is that of a stable sort of the keys (ties keep
+ * index order). Every particle array of `src` is gathered into `dst` in
+ * sorted order and ids_dst[t] = ids_src[perm[t]]; keys_out / perm_out (n
+ * each, optional) receive the sorted keys and the permutation. With
+ * `chunks` (2 * max_chunks int32, optional) it also writes the P2G work list
+ * of the sorted order: the particles of each 4x4x4 cell block (and the
+ * inactive ones) cut into chunks of at most 256, and their number to the
+ * device int *chunk_count (at most num_particles/256 + occupied blocks + 1;
+ * an error when max_chunks is smaller). */
+int fsi_mpm_sort(const fsi_mpm_params* p, const fsi_mpm_buffers* src, const fsi_mpm_buffers* dst,
+                 const int32_t* ids_src, int32_t* ids_dst, void* workspace, int64_t workspace_bytes,
+                 uint32_t* keys_out, int32_t* perm_out, int32_t* chunks, int32_t max_chunks,
+                 int32_t* chunk_count, void* stream);
 
 /* fsi/coupling.py:463-496 alone (used by compute_coupling_forces). */
 int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,

@@ -330,13 +330,23 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
                                                 R* __restrict__ valid_weight,
                                                 int32_t* __restrict__ cmask,
                                                 int32_t* __restrict__ counters, int clear_force,
-                                                int32_t* __restrict__ zero_ctr) {
+                                                int32_t* __restrict__ zero_ctr,
+                                                const int32_t* __restrict__ chunks) {
   __shared__ Tile::acc tile[Tile::kNodes * Tile::kValues];
   // batched substeps: the G2P-pass counters of this substep start at zero
   if (zero_ctr && blockIdx.x == 0 && threadIdx.x < 2) zero_ctr[threadIdx.x] = 0;
   __shared__ int red[6][kBlock / 64];
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = p < m.num;
+  // particle of this lane: from the work list (one chunk of at most 256
+  // particles of one 4x4x4 cell block per workgroup), else slot order
+  int p;
+  bool live;
+  if (chunks) {
+    p = chunks[2 * blockIdx.x] + threadIdx.x;
+    live = (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
+  } else {
+    p = blockIdx.x * blockDim.x + threadIdx.x;
+    live = p < m.num;
+  }
   const bool act = live && active[p] != 0;
   const int cap = m.cap;
   const int64_t N = (int64_t)m.nx * m.ny * m.nz;
@@ -866,13 +876,15 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
     FSI_TRY_LAUNCH("zero_counters");
   }
   if (m.num == 0) return 0;
-  const dim3 grid(blocks_for(m.num)), block(kBlock);
+  // with a work list from the particle sort, one workgroup per chunk
+  const int32_t* chunks = (b->chunks && p->num_chunks > 0) ? b->chunks : nullptr;
+  const dim3 grid(chunks ? (unsigned)p->num_chunks : blocks_for(m.num)), block(kBlock);
 #define FSI_P2G_ARGS                                                                            \
   m, cp, (R*)b->x, (R*)b->v, (const R*)b->C, (R*)b->F, (const R*)b->mass, (const R*)b->volume, \
       b->active, (R*)b->pforce, grid_target, cb ? cb->solid : nullptr,                          \
       cb ? (const R*)cb->lbm_vel : nullptr, cb ? (R*)cb->cforce : nullptr,                      \
       cb ? (R*)cb->valid_weight : nullptr, cb ? cb->coupling_mask : nullptr,                    \
-      cb ? cb->counters : nullptr, clear_force, zero_ctr
+      cb ? cb->counters : nullptr, clear_force, zero_ctr, chunks
   if (couple && do_p2g)
     hipLaunchKernelGGL((k_p2g<R, true, true>), grid, block, 0, st, FSI_P2G_ARGS);
   else if (couple)

@@ -59,12 +59,13 @@ class MpmParams(ctypes.Structure):
         ("boundary_damping", ctypes.c_double),
         ("apic", ctypes.c_int32),
         ("clamp_lower", ctypes.c_double), ("clamp_upper", c_double3),
+        ("num_chunks", ctypes.c_int32),
     ]
 
 
 class MpmBuffers(ctypes.Structure):
     _fields_ = [("x", _vp), ("v", _vp), ("C", _vp), ("F", _vp), ("mass", _vp), ("volume", _vp),
-                ("active", _vp), ("pforce", _vp), ("grid", _vp), ("grid_alt", _vp)]
+                ("active", _vp), ("pforce", _vp), ("grid", _vp), ("grid_alt", _vp), ("chunks", _vp)]
 
 
 class CouplingParams(ctypes.Structure):
@@ -110,6 +111,7 @@ _SIGNATURES = {
     "fsi_contact": [_vp, _vp, _vp, _vp, _vp],
     "fsi_polar_rotation": [ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp],
     "fsi_force_assemble": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp],
+    "fsi_mpm_sort": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int32, _vp, _vp],
     "fsi_reduce_lattice": [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
     "fsi_reduce_particles": [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
     "fsi_abi_version": [],
@@ -135,6 +137,8 @@ def load_library() -> ctypes.CDLL:
             fn = getattr(lib, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int
+        lib.fsi_mpm_sort_workspace.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
+        lib.fsi_mpm_sort_workspace.restype = ctypes.c_int
         lib.fsi_last_error.argtypes = []
         lib.fsi_last_error.restype = ctypes.c_char_p
         _lib = lib

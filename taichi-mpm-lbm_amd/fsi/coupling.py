@@ -64,6 +64,11 @@ class LBMMpmCoupler:
         self.contact_mask = torch.zeros(cap, dtype=torch.int32, device=dev)
         self.counters = torch.zeros(8, dtype=torch.int32, device=dev)
         self.coupled_step = 0
+        # the particles are re-sorted by cell key every `sort_interval`
+        # coupled steps (0: never), keeping each P2G workgroup's particles in
+        # a compact box; results change only by float rounding of the sums
+        self.sort_interval = 1
+        self._steps_since_sort = 0
         self._cbuf = nat.CouplingBuffers(
             nat.ptr(lbm.solid), nat.ptr(lbm.vel), nat.ptr(self.cforce), nat.ptr(self.valid_weight),
             nat.ptr(self.coupling_mask), nat.ptr(self.contact_mask), nat.ptr(self.counters),
@@ -149,6 +154,9 @@ class LBMMpmCoupler:
         self._validate_initialized()
 
         self.lbm._ensure_macro()  # the substeps interpolate the lattice velocity of this state
+        if self.sort_interval > 0 and self._steps_since_sort % self.sort_interval == 0:
+            self.mpm.sort_particles()
+        self._steps_since_sort += 1
         self.clear_coupling_fields()
         mpm_dt = lbm_dt_value / float(self.substeps)
         dt_ratio = mpm_dt / lbm_dt_value
@@ -226,14 +234,18 @@ class LBMMpmCoupler:
                  nat.ptr(self.ib_force), nat.ptr(out))
         return self._cells(out, 3)
 
+    def _per_particle(self, t: torch.Tensor) -> np.ndarray:
+        # slot order -> input order (the particles may have been sorted)
+        return self.mpm._input_order(t[: self.mpm.particle_count()].cpu().numpy().copy())
+
     def particle_valid_weights_numpy(self) -> np.ndarray:
-        return self.valid_weight[: self.mpm.particle_count()].cpu().numpy().copy()
+        return self._per_particle(self.valid_weight)
 
     def particle_coupling_mask_numpy(self) -> np.ndarray:
-        return self.coupling_mask[: self.mpm.particle_count()].cpu().numpy().copy()
+        return self._per_particle(self.coupling_mask)
 
     def particle_contact_mask_numpy(self) -> np.ndarray:
-        return self.contact_mask[: self.mpm.particle_count()].cpu().numpy().copy()
+        return self._per_particle(self.contact_mask)
 
     # The accessors keep the reference's float32 host reductions;
     # diagnostic_vector() gathers the FP64 device reductions the per-step

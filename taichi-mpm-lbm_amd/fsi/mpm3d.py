@@ -9,6 +9,8 @@ three launches: grid clear, P2G, grid update + G2P.
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -65,6 +67,12 @@ class MPMSolver3D:
         # zeroing the one the next P2G pass fills
         self.grid_alt = torch.zeros_like(self.grid)
         self._sum_ws, self._sum_out = nat.reduction_buffers(dev)
+        # particle order: slot t holds input particle ids[t]; sort_particles()
+        # reorders the slots by cell key (buffers for it are made on first use)
+        self.ids = torch.arange(cap, dtype=torch.int32, device=dev)
+        self._sort = None
+        self._permuted = False
+        self._num_chunks = 0  # P2G work list length after a sort (0: slot order)
         self.num_particles = 0
         self._buffers = nat.MpmBuffers(
             nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
@@ -161,6 +169,9 @@ class MPMSolver3D:
         self.volume.copy_(torch.from_numpy(vol))
         self.active.copy_(torch.from_numpy(active))
         nat.fill_zero(self.particle_force)
+        self.ids.copy_(torch.arange(cap, dtype=torch.int32))
+        self._permuted = False
+        self._num_chunks = 0
         self.num_particles = count
         self._initialized = True
         return count
@@ -205,6 +216,7 @@ class MPMSolver3D:
         p.boundary_width = self.boundary_width
         p.boundary_damping = self.boundary_damping
         p.apic = 1 if self.apic else 0
+        p.num_chunks = self._num_chunks if self._permuted else 0
         p.clamp_lower = 0.5 * self.dx
         p.clamp_upper = nat.c_double3((self.nx - 1.5) * self.dx, (self.ny - 1.5) * self.dx,
                                       (self.nz - 1.5) * self.dx)
@@ -229,6 +241,82 @@ class MPMSolver3D:
         nat.call("fsi_mpm_p2g", nat.ref(p), nat.ref(self._buffers), None, None, 0, 1)
         nat.call("fsi_mpm_grid_g2p", nat.ref(p), nat.ref(self._buffers), None, None)
 
+    def _sort_state(self):
+        if self._sort is None:
+            cap, dev, rd = self.max_particles, self.device, self.real_dtype
+            alt = {
+                "x": torch.empty(3 * cap, dtype=rd, device=dev),
+                "v": torch.empty(3 * cap, dtype=rd, device=dev),
+                "C": torch.empty(9 * cap, dtype=rd, device=dev),
+                "F": torch.empty(9 * cap, dtype=rd, device=dev),
+                "mass": torch.empty(cap, dtype=rd, device=dev),
+                "volume": torch.empty(cap, dtype=rd, device=dev),
+                "active": torch.zeros(cap, dtype=torch.int32, device=dev),
+                "particle_force": torch.empty(3 * cap, dtype=rd, device=dev),
+                "ids": torch.empty(cap, dtype=torch.int32, device=dev),
+            }
+            nbytes = ctypes.c_int64(0)
+            p = self.params(self.dt_default)
+            rc = nat.load_library().fsi_mpm_sort_workspace(nat.ref(p), ctypes.byref(nbytes))
+            if rc != 0:
+                raise RuntimeError(nat.load_library().fsi_last_error().decode(errors="replace"))
+            ws = torch.empty(max(int(nbytes.value), 1), dtype=torch.uint8, device=dev)
+            # P2G work list: at most cap/256 + (cell blocks + 1) chunks
+            blocks = ((self.nx + 5) // 4) * ((self.ny + 5) // 4) * ((self.nz + 5) // 4) + 1
+            max_chunks = cap // 256 + min(blocks, cap) + 1
+            chunks = torch.zeros(2 * max_chunks, dtype=torch.int32, device=dev)
+            count = torch.zeros(1, dtype=torch.int32, device=dev)
+            self._sort = {"alt": alt, "ws": ws, "chunks": chunks, "max_chunks": max_chunks, "count": count}
+        return self._sort
+
+    _SORTED_FIELDS = ("x", "v", "C", "F", "mass", "volume", "active", "particle_force")
+
+    def _make_buffers(self) -> nat.MpmBuffers:
+        chunks = self._sort["chunks"] if self._sort is not None else None
+        return nat.MpmBuffers(
+            nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
+            nat.ptr(self.mass), nat.ptr(self.volume), nat.ptr(self.active),
+            nat.ptr(self.particle_force), nat.ptr(self.grid), nat.ptr(self.grid_alt), nat.ptr(chunks),
+        )
+
+    def sort_particles(self, keys_out: torch.Tensor | None = None,
+                       perm_out: torch.Tensor | None = None) -> None:
+        """Reorder the particle slots by cell key (a stable counting sort on
+        the device; see fsi_mpm_sort). Results do not depend on the order
+        beyond float rounding of the grid sums; the accessors return the
+        particles in input order whatever the slot order."""
+        if not self._initialized:
+            raise RuntimeError("MPMSolver3D has no particles to sort.")
+        st = self._sort_state()
+        alt = st["alt"]
+        dst = nat.MpmBuffers(
+            nat.ptr(alt["x"]), nat.ptr(alt["v"]), nat.ptr(alt["C"]), nat.ptr(alt["F"]),
+            nat.ptr(alt["mass"]), nat.ptr(alt["volume"]), nat.ptr(alt["active"]),
+            nat.ptr(alt["particle_force"]), nat.ptr(self.grid), nat.ptr(self.grid_alt),
+        )
+        p = self.params(self.dt_default)
+        nat.call("fsi_mpm_sort", nat.ref(p), nat.ref(self._buffers), nat.ref(dst), nat.ptr(self.ids),
+                 nat.ptr(alt["ids"]), nat.ptr(st["ws"]), st["ws"].numel(), nat.ptr(keys_out),
+                 nat.ptr(perm_out), nat.ptr(st["chunks"]), st["max_chunks"], nat.ptr(st["count"]))
+        for name in self._SORTED_FIELDS + ("ids",):
+            cur = getattr(self, name)
+            setattr(self, name, alt[name])
+            alt[name] = cur
+        self._buffers = self._make_buffers()
+        self._permuted = True
+        # the launch size of P2G needs the work list length on the host
+        self._num_chunks = int(st["count"].item())
+
+    def _input_order(self, arr: np.ndarray) -> np.ndarray:
+        """Rows of a slot-ordered host array (first particle_count() slots)
+        put back in input order."""
+        if not self._permuted:
+            return arr
+        ids = self.ids[: self.num_particles].cpu().numpy()
+        out = np.empty_like(arr)
+        out[ids] = arr
+        return out
+
     def clear_particle_force(self) -> None:
         # fsi/mpm3d.py:211-214
         nat.fill_zero(self.particle_force)
@@ -237,7 +325,7 @@ class MPMSolver3D:
 
     def _plane(self, t: torch.Tensor, ncomp: int) -> np.ndarray:
         n = self.num_particles
-        return t.view(ncomp, self.max_particles)[:, :n].cpu().numpy().T.copy()
+        return self._input_order(t.view(ncomp, self.max_particles)[:, :n].cpu().numpy().T.copy())
 
     def particle_count(self) -> int:
         return int(self.num_particles)
@@ -255,10 +343,10 @@ class MPMSolver3D:
         return self._plane(self.F, 9).reshape(-1, 3, 3)
 
     def active_numpy(self) -> np.ndarray:
-        return self.active[: self.num_particles].cpu().numpy().copy()
+        return self._input_order(self.active[: self.num_particles].cpu().numpy().copy())
 
     def masses_numpy(self) -> np.ndarray:
-        return self.mass[: self.num_particles].cpu().numpy().copy()
+        return self._input_order(self.mass[: self.num_particles].cpu().numpy().copy())
 
     def particle_sums(self, valid_weight: torch.Tensor | None = None) -> torch.Tensor:
         """FP64 sums over the first particle_count() particles, on the device

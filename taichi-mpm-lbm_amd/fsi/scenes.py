@@ -103,3 +103,38 @@ def random_rotation(rng):
     if np.linalg.det(q) < 0:
         q[:, 0] *= -1.0
     return q
+
+
+def sphere_pack(shape, count, rmin, rmax, spacing=0.5, seed=0, margin=3.0, gap=1.0, jitter=0.5,
+                max_tries=100_000):
+    """Particles of `count` non-overlapping spheres in a box of `shape` cells.
+
+    Radii are U[rmin, rmax]; centres U over the box shrunk by radius + margin
+    on every face; a candidate is rejected when it comes closer than `gap`
+    cells to an accepted sphere. Each sphere is a jittered lattice of the
+    given spacing (seeded per sphere). Particles are returned sphere by
+    sphere, each in lattice order.
+    Returns (positions float32 (n, 3), info dict with centres, radii, counts).
+    """
+    rng = np.random.default_rng(seed)
+    shape = np.asarray(shape, float)
+    centres, radii = [], []
+    tries = 0
+    while len(centres) < count:
+        tries += 1
+        if tries > max_tries:
+            raise ValueError(f"could not place {count} spheres in {tuple(shape)} after {max_tries} tries")
+        r = rng.uniform(rmin, rmax)
+        lo, hi = r + margin, shape - r - margin
+        if np.any(hi <= lo):
+            continue
+        c = rng.uniform(lo, hi)
+        if all(np.linalg.norm(c - c2) >= r + r2 + gap for c2, r2 in zip(centres, radii)):
+            centres.append(c)
+            radii.append(r)
+    parts = [jittered_sphere(c, r, spacing, seed=seed * 1000 + i, jitter=jitter)
+             for i, (c, r) in enumerate(zip(centres, radii))]
+    pos = np.concatenate(parts, axis=0).astype(np.float32)
+    info = {"centres": np.array(centres), "radii": np.array(radii), "counts": [len(p) for p in parts]}
+    return pos, info
+
