@@ -253,16 +253,63 @@ __device__ __forceinline__ void rotation_core(const R F[9], R Rot[9], R* lam_min
       Rot[r * 3 + s] = Qm[r][0] * V[s][0] + Qm[r][1] * V[s][1] + Qm[r][2] * V[s][2];
 }
 
-// FP32 rotation with an FP64 recomputation for near-degenerate F: when the
-// squared singular values span more than 1e5 (F^T F loses the small one in
-// FP32) or F is inverted.
+// Rotation factor of F: FP64 builds use the Jacobi SVD above; FP32 uses the
+// Newton polar iteration and recomputes in FP64 (Jacobi) for inverted or
+// ill-conditioned F.
+// Orthogonal polar factor of F by the determinant-scaled Newton iteration
+// X <- (g X + (g X)^-T) / 2, g = |det X|^(-1/3) (N. J. Higham, "Computing
+// the polar decomposition - with applications", SIAM J. Sci. Stat. Comput. 7
+// (1986) 1160-1174). For det F > 0 its limit is the polar factor, which is
+// R = U V^T of the rotation-variant SVD. X^-T = cof(X) / det X. Returns false
+// (and leaves Rot unspecified) when det F <= 0 or the Frobenius condition
+// number of F exceeds 300 (the squared singular values may then span more
+// than 1e5); the caller then takes the FP64 Jacobi path.
+__device__ __forceinline__ bool polar_newton(const float F[9], float X[9]) {
+  constexpr int kMaxIter = 10;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) X[i] = F[i];
+  for (int it = 0; it < kMaxIter; ++it) {
+    float c[9];
+    c[0] = X[4] * X[8] - X[5] * X[7];
+    c[1] = X[5] * X[6] - X[3] * X[8];
+    c[2] = X[3] * X[7] - X[4] * X[6];
+    c[3] = X[2] * X[7] - X[1] * X[8];
+    c[4] = X[0] * X[8] - X[2] * X[6];
+    c[5] = X[1] * X[6] - X[0] * X[7];
+    c[6] = X[1] * X[5] - X[2] * X[4];
+    c[7] = X[2] * X[3] - X[0] * X[5];
+    c[8] = X[0] * X[4] - X[1] * X[3];
+    const float det = X[0] * c[0] + X[1] * c[1] + X[2] * c[2];
+    if (it == 0) {
+      float nf = 0.0f, nc = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        nf += X[i] * X[i];
+        nc += c[i] * c[i];
+      }
+      // kappa_F = |F|_F |F^-1|_F = |F|_F |cof F|_F / det
+      if (!(det > 0.0f) || !(nf * nc <= 9.0e4f * det * det)) return false;
+    }
+    const float g = exp2f(-log2f(det) * (1.0f / 3.0f));
+    const float a = 0.5f * g, b = 0.5f / (g * det);
+    float diff = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const float xn = a * X[i] + b * c[i];
+      diff = fmaxf(diff, fabsf(xn - X[i]));
+      X[i] = xn;
+    }
+    if (diff <= 1.0e-6f) break;
+  }
+  return true;
+}
+
 template <typename R> __device__ __forceinline__ void rotation_factor(const R F[9], R Rot[9], R J) {
   if constexpr (sizeof(R) == 8) {
     rotation_core<R>(F, Rot, nullptr);
   } else {
-    R ratio;
-    rotation_core<R>(F, Rot, &ratio);
-    if (!(ratio > R(1e-5)) || !(J > R(0))) {
+    (void)J;
+    if (!polar_newton(F, Rot)) {
       double Fd[9], Rd[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) Fd[i] = double(F[i]);
@@ -300,7 +347,10 @@ __global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf
 struct Tile {
   using acc = double;
   static constexpr int kValues = 8;
-  static constexpr int kNodes = (64 * 1024) / (kValues * (int)sizeof(acc));
+  // 32 KB: a work-list chunk (one 4x4x4 cell block) spans at most ~6^3
+  // nodes plus drift since the last sort; 32 KB per workgroup leaves LDS for
+  // four workgroups per CU
+  static constexpr int kNodes = (32 * 1024) / (kValues * (int)sizeof(acc));
 };
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -356,8 +406,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
   R xp[3] = {R(0), R(0), R(0)}, vp[3] = {R(0), R(0), R(0)}, fx[3] = {R(0), R(0), R(0)}, w[3][3];
   R Cm[9], Fo[9], mp = R(0), vp_vol = R(0);
   int base[3] = {0, 0, 0};
-  // every per-particle load is issued here, ahead of the node loops, so the
-  // wave waits on memory once
+  // per-particle state used by both parts (C and F are loaded where the
+  // P2G part needs them, keeping registers free during the coupling gather)
   if (act) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -366,13 +416,6 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
     }
     mp = mass[p];
     vp_vol = vol[p];
-    if constexpr (P2G) {
-#pragma unroll
-      for (int r = 0; r < 9; ++r) {
-        Cm[r] = C[r * cap + p];
-        Fo[r] = Fd[r * cap + p];
-      }
-    }
     locate(xp, m.inv_dx, base, fx);
   }
   bspline(fx, w);
@@ -438,44 +481,44 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
       cmask[p] = 0;
     }
     if (act) {
-      // load phase: the 27 nodes' solid flags and fluid velocities are all in
-      // flight before the first is used
-      int32_t sn[27];
-      R un[27][3];
-      uint32_t inmask = 0;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
+      // support of the valid (in-lattice, fluid) nodes and the interpolated
+      // lattice velocity, in i, j, k order; the nodes are gathered one
+      // i-plane (9 nodes, all loads in flight together) at a time
+      R wsum = R(0), u[3] = {R(0), R(0), R(0)};
+      uint32_t valid = 0;  // bit q = (i*3+j)*3+k
+#pragma unroll 1
+      for (int i = 0; i < 3; ++i) {
+        const R wi = pick3(i, w[0][0], w[1][0], w[2][0]);
+        int32_t sn[9];
+        R un[9][3];
+        uint32_t inmask = 0;
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
             bool in;
             const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
-            const int q = (i * 3 + j) * 3 + k;
+            const int q = j * 3 + k;
             sn[q] = solid[node];
             un[q][0] = lbm_vel[node];
             un[q][1] = lbm_vel[N + node];
             un[q][2] = lbm_vel[2 * N + node];
             inmask |= (uint32_t)in << q;
           }
-      // support of the valid (in-lattice, fluid) nodes, q ascending
-      R wsum = R(0), u[3] = {R(0), R(0), R(0)};
-      uint32_t valid = 0;  // bit q = (i*3+j)*3+k
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            const int q = (i * 3 + j) * 3 + k;
+            const int q = j * 3 + k;
             const bool ok = ((inmask >> q) & 1u) && sn[q] == 0;
-            valid |= (uint32_t)ok << q;
-            const R wt = w[i][0] * w[j][1] * w[k][2];
+            valid |= (uint32_t)ok << (i * 9 + q);
+            const R wt = wi * w[j][1] * w[k][2];
             wsum += ok ? wt : R(0);
             u[0] += ok ? wt * un[q][0] : R(0);
             u[1] += ok ? wt * un[q][1] : R(0);
             u[2] += ok ? wt * un[q][2] : R(0);
           }
+      }
       valid_weight[p] = wsum;
       if (wsum > cp.min_w) {
         partial = wsum < R(1) - R(1e-5);
@@ -503,6 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
         }
         cmask[p] = 1;
         // equal and opposite reaction (and occupancy) spread on the valid nodes
+        const R inv_wsum = R(1) / wsum;  // one division, 27 products below
         const R fscale = cp.dt_ratio / cp.cell_volume;
         const R phi = vp_vol / cp.cell_volume;
         const R rf[3] = {-pf[0] * fscale, -pf[1] * fscale, -pf[2] * fscale};
@@ -518,7 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
             const bool ok = (valid >> q) & 1u;
             const R wt = pick3(i, w[0][0], w[1][0], w[2][0]) * pick3(j, w[0][1], w[1][1], w[2][1]) *
                          pick3(k, w[0][2], w[1][2], w[2][2]);
-            const R nv = ok ? wt / wsum : R(0);
+            const R nv = ok ? wt * inv_wsum : R(0);
             const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
             atomicAdd(&FSI_TILE(ni, nj, nk, 4), nv * rf[0]);
             atomicAdd(&FSI_TILE(ni, nj, nk, 5), nv * rf[1]);
@@ -535,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
               for (int k = 0; k < 3; ++k) {
                 if (!((valid >> ((i * 3 + j) * 3 + k)) & 1u)) continue;
                 const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-                const R nw = (w[i][0] * w[j][1] * w[k][2]) / wsum;
+                const R nw = (w[i][0] * w[j][1] * w[k][2]) * inv_wsum;
                 const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
                 FSI_GLOBAL(node, 4, nw * rf[0]);
                 FSI_GLOBAL(node, 5, nw * rf[1]);
@@ -570,6 +614,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
   if constexpr (P2G) {
     if (act) {
       const R dt = m.dt;
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        Cm[r] = C[r * cap + p];
+        Fo[r] = Fd[r * cap + p];
+      }
       R Fn[9];
       // F <- (I + dt C) F
 #pragma unroll
@@ -676,11 +725,12 @@ __device__ __forceinline__ void grid_velocity(const MpmK<R>& m, const R g[4], in
   // branch-free: nodes without mass keep their (zero) momentum
   const R mass = g[3];
   const bool has = mass > R(0);
+  const R inv_mass = R(1) / mass;  // one division per node (massless nodes: unused)
   const int idx[3] = {i, j, k};
   const int n[3] = {m.nx, m.ny, m.nz};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    R va = g[a] / mass + m.dt * m.g[a];
+    R va = g[a] * inv_mass + m.dt * m.g[a];
     const bool flip = (idx[a] < m.bw && va < R(0)) || (idx[a] >= n[a] - m.bw && va > R(0));
     va = flip ? -m.damping * va : va;
     out[a] = has ? va : g[a];
@@ -746,11 +796,15 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
     bspline(fx, w);
     R nv[3] = {R(0), R(0), R(0)};
     R nc[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
-    // all 27 nodes are loaded before any is used: one memory round trip
-    R gn[27][4];
-    uint32_t inmask = 0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
+    // nodes are gathered one i-plane (9 nodes) at a time: the 9 loads of a
+    // plane are in flight together, and the registers they need leave room
+    // for more wavefronts per SIMD than a 27-node batch. Sums keep the
+    // i, j, k order.
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      const R wi = pick3(i, w[0][0], w[1][0], w[2][0]);
+      R gn[9][4];
+      uint32_t inmask = 0;
 #pragma unroll
       for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -758,34 +812,33 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
           bool in;
           const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
           const R* gp = grid + 4 * node;
-          const int q = (i * 3 + j) * 3 + k;
+          const int q = j * 3 + k;
 #pragma unroll
           for (int a = 0; a < 4; ++a) gn[q][a] = gp[a];
           inmask |= (uint32_t)in << q;
         }
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
       for (int j = 0; j < 3; ++j)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const int q = (i * 3 + j) * 3 + k;
+          const int q = j * 3 + k;
           const bool in = (inmask >> q) & 1u;
-          const R* g4 = gn[q];
           R gv[3];
-          grid_velocity(m, g4, base[0] + i, base[1] + j, base[2] + k, gv);
-          const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
-          const R wt = w[i][0] * w[j][1] * w[k][2];
-          // out-of-lattice nodes contribute exact zeros through selects (no
-          // branch, so all 27 nodes' loads are in flight together)
+          grid_velocity(m, gn[q], base[0] + i, base[1] + j, base[2] + k, gv);
+          // out-of-lattice nodes contribute exact zeros
 #pragma unroll
-          for (int a = 0; a < 3; ++a) nv[a] += in ? wt * gv[a] : R(0);
+          for (int a = 0; a < 3; ++a) gv[a] = in ? gv[a] : R(0);
+          const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
+          const R wt = wi * w[j][1] * w[k][2];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) nv[a] += wt * gv[a];
           const R s = m.four_inv_dx * wt;
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) nc[r * 3 + c] += in ? s * (gv[r] * dp[c]) : R(0);
+            for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
         }
+    }
 #pragma unroll
     for (int r = 0; r < 9; ++r) C[r * cap + p] = m.apic ? nc[r] : R(0);
 #pragma unroll
