@@ -29,16 +29,42 @@ __all__ = [
     "MPMConfig",
     "MPMSolver3D",
     "OutputConfig",
+    "ReferenceDataset",
+    "ReferenceReport",
     "SimulationConfig",
     "SimulationOutputWriter",
+    "SnapshotInfo",
+    "ValidationMetric",
+    "ValidationReport",
+    "extract_snapshot_timeseries",
+    "list_npz_snapshots",
+    "load_reference_dataset",
+    "load_validation_summary",
+    "run_reference_validation_suite",
+    "run_validation_suite",
+    "validation_summary_table",
 ]
 
+# name -> (module, attribute); the solver modules load torch and the kernel
+# library, the others only numpy
 _LAZY = {
     "LBMSolver3D": ("lbm3d", "LBMSolver3D"),
     "MPMSolver3D": ("mpm3d", "MPMSolver3D"),
     "LBMMpmCoupler": ("coupling", "LBMMpmCoupler"),
     "FSISimulation": ("simulation", "FSISimulation"),
     "SimulationOutputWriter": ("output", "SimulationOutputWriter"),
+    "SnapshotInfo": ("postprocess", "SnapshotInfo"),
+    "extract_snapshot_timeseries": ("postprocess", "extract_snapshot_timeseries"),
+    "list_npz_snapshots": ("postprocess", "list_npz_snapshots"),
+    "load_validation_summary": ("postprocess", "load_validation_summary"),
+    "validation_summary_table": ("postprocess", "validation_summary_table"),
+    "ReferenceDataset": ("reference", "ReferenceDataset"),
+    "ReferenceReport": ("reference", "ReferenceReport"),
+    "load_reference_dataset": ("reference", "load_reference_dataset"),
+    "ValidationMetric": ("validation", "ValidationMetric"),
+    "ValidationReport": ("validation", "ValidationReport"),
+    "run_validation_suite": ("validation_cases", "run_validation_suite"),
+    "run_reference_validation_suite": ("reference_cases", "run_reference_validation_suite"),
 }
 
 
