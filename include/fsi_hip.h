@@ -161,7 +161,7 @@ typedef struct fsi_mpm_buffers {
                               substep needs no separate clear launch */
   const int32_t* chunks;   /* optional P2G work list from fsi_mpm_sort: (first slot,
                               count <= 256) pairs, one workgroup each; every chunk
-                              lies in one 4x4x4 cell block */
+                              lies in a 4x4x8 cell box */
 } fsi_mpm_buffers;
 
 /* Coupling inputs/outputs read and written inside the particle passes
@@ -228,14 +228,15 @@ int fsi_mpm_sort_workspace(const fsi_mpm_params* p, int64_t* bytes);
  * counterpart in the reference, whose particles keep their input order; it
  * changes only the order in which particles are processed). Key: the
  * truncating base cell of fsi/mpm3d.py:239 (computed as P2G computes it),
- * clamped to [-1, n] per axis, in 4x4x4-blocked order; inactive particles
- * last. The permutation is that of a stable sort of the keys (ties keep
+ * clamped to [-1, n] per axis, in 4x4x4-blocked order (blocks row-major,
+ * an even number of blocks in z); inactive particles last. The permutation is that of a stable sort of the keys (ties keep
  * index order). Every particle array of `src` is gathered into `dst` in
  * sorted order and ids_dst[t] = ids_src[perm[t]]; keys_out / perm_out (n
  * each, optional) receive the sorted keys and the permutation. With
  * `chunks` (2 * max_chunks int32, optional) it also writes the P2G work list
- * of the sorted order: the particles of each 4x4x4 cell block (and the
- * inactive ones) cut into chunks of at most 256, and their number to the
+ * of the sorted order: the particles of each pair of z-adjacent 4x4x4 cell
+ * blocks (and the inactive ones) cut into chunks of at most 256, and their
+ * number to the
  * device int *chunk_count (at most num_particles/256 + occupied blocks + 1;
  * an error when max_chunks is smaller). */
 int fsi_mpm_sort(const fsi_mpm_params* p, const fsi_mpm_buffers* src, const fsi_mpm_buffers* dst,

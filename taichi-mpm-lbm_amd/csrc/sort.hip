@@ -2,9 +2,11 @@
 //
 // Key: the particle's base cell (the truncating key of fsi/mpm3d.py:239,
 // computed exactly as P2G computes it), shifted by one and clamped to
-// [0, n+1] per axis, in 4x4x4-blocked order, so the 256 particles of one P2G
-// workgroup cover a compact box (about 4x4x2 cells at 8 particles per cell)
-// and their LDS tile stays small. Inactive particles take the last key.
+// [0, n+1] per axis, in 4x4x4-blocked order (blocks row-major, z fastest,
+// an even number of blocks in z). The P2G work list cuts the particles of
+// each pair of z-adjacent blocks into chunks of at most 256, so a chunk's
+// stencil box is at most 6x6x10 nodes and its LDS tile stays small.
+// Inactive particles take the last key.
 //
 // Sort: histogram -> exclusive scan -> scatter with an atomic cursor per key
 // -> each key's segment ordered by particle index (ties keep index order),
@@ -23,7 +25,7 @@ constexpr int kInsertionMax = 32;                  // longer key segments use he
 
 struct SortGeom {
   int nx, ny, nz;           // lattice cells
-  int pbx, pby, pbz;        // 4-cell blocks per axis of the padded (n+2) box
+  int pbx, pby, pbz;        // 4-cell blocks per axis of the padded (n+2) box (pbz even)
   uint32_t nkeys;           // keys of cells; key nkeys = inactive particles
 };
 
@@ -34,7 +36,9 @@ SortGeom make_geom(const fsi_mpm_params* p) {
   g.nz = p->nz;
   g.pbx = (p->nx + 2 + 3) / 4;
   g.pby = (p->ny + 2 + 3) / 4;
-  g.pbz = (p->nz + 2 + 3) / 4;
+  // an even number of z blocks: the block pairs of a P2G chunk group are
+  // then always neighbours in z within one row
+  g.pbz = ((p->nz + 2 + 3) / 4 + 1) & ~1;
   g.nkeys = (uint32_t)((int64_t)g.pbx * g.pby * g.pbz * 64);
   return g;
 }
@@ -54,12 +58,18 @@ struct SortWs {
 
 int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
 
+constexpr int64_t kGroupKeysHost = 128;  // keys per P2G chunk group (see k_chunk_counts)
+
+// chunk groups for nk = nkeys + 1 keys: ceil(nkeys / group) cell groups and
+// the inactive bucket
+int64_t chunk_groups(int64_t nk) { return (nk - 1 + kGroupKeysHost - 1) / kGroupKeysHost + 1; }
+
 SortWs carve(void* base, int64_t nk, int64_t n) {
   SortWs w;
   char* c = (char*)base;
   int64_t off = 0;
   const int64_t nblk = (nk + kScanBlock - 1) / kScanBlock;  // also covers the cell-block scan
-  const int64_t ncb = (nk - 1) / 64 + 1;                     // cell blocks + inactive bucket
+  const int64_t ncb = chunk_groups(nk);                      // chunk groups + inactive bucket
   w.counts = (int32_t*)(c + off);
   off += align256(nk * 4);
   w.starts = (int32_t*)(c + off);
@@ -237,32 +247,44 @@ __global__ __launch_bounds__(kBlock) void k_order_segments(int64_t nk, const int
   }
 }
 
-// P2G work list: block b < ncells_blocks holds keys [64b, 64b+64), block
-// ncb-1 the inactive bucket; its particles [lo, hi) of the sorted order are
-// cut into chunks of at most kBlock.
-__device__ __forceinline__ void block_range(int64_t b, int64_t ncb, int num, const int32_t* __restrict__ starts,
-                                            int& lo, int& hi) {
-  lo = starts[b * 64];
-  hi = b + 1 < ncb ? starts[(b + 1) * 64] : num;
+// P2G work list. Group g < ncb-1 holds the keys [g*kGroupKeys,
+// min((g+1)*kGroupKeys, nkeys)): two 4x4x4 cell blocks adjacent in z, so a
+// chunk's cells lie in a 4x4x8 box (stencil box 6x6x10 = 360 nodes, inside
+// the 512-node P2G tile); group ncb-1 is the inactive bucket (key nkeys).
+// Each group's particles [lo, hi) of the sorted order are cut into chunks of
+// at most kBlock.
+constexpr int64_t kGroupKeys = kGroupKeysHost;
+
+__device__ __forceinline__ void block_range(int64_t b, int64_t ncb, int64_t nkeys, int num,
+                                            const int32_t* __restrict__ starts, int& lo, int& hi) {
+  if (b + 1 < ncb) {
+    lo = starts[b * kGroupKeys];
+    hi = starts[min((b + 1) * kGroupKeys, nkeys)];
+  } else {
+    lo = starts[nkeys];
+    hi = num;
+  }
 }
 
-__global__ __launch_bounds__(kBlock) void k_chunk_counts(int64_t ncb, int num, const int32_t* __restrict__ starts,
+__global__ __launch_bounds__(kBlock) void k_chunk_counts(int64_t ncb, int64_t nkeys, int num,
+                                                        const int32_t* __restrict__ starts,
                                                         int32_t* __restrict__ nchunk) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ncb) return;
   int lo, hi;
-  block_range(b, ncb, num, starts, lo, hi);
+  block_range(b, ncb, nkeys, num, starts, lo, hi);
   nchunk[b] = (hi - lo + kBlock - 1) / kBlock;
 }
 
-__global__ __launch_bounds__(kBlock) void k_chunk_write(int64_t ncb, int num, const int32_t* __restrict__ starts,
+__global__ __launch_bounds__(kBlock) void k_chunk_write(int64_t ncb, int64_t nkeys, int num,
+                                                       const int32_t* __restrict__ starts,
                                                        const int32_t* __restrict__ nchunk,
                                                        const int32_t* __restrict__ chunk_off, int max_chunks,
                                                        int32_t* __restrict__ chunks, int32_t* __restrict__ total) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ncb) return;
   int lo, hi;
-  block_range(b, ncb, num, starts, lo, hi);
+  block_range(b, ncb, nkeys, num, starts, lo, hi);
   const int o = chunk_off[b];
   for (int c = 0; c < nchunk[b]; ++c) {
     if (o + c >= max_chunks) break;
@@ -350,13 +372,15 @@ int sort_t(const fsi_mpm_params* p, const fsi_mpm_buffers* src, const fsi_mpm_bu
                      perm_out);
   FSI_TRY_LAUNCH("mpm_sort gather");
   if (chunks) {
-    const int64_t ncb = (nk - 1) / 64 + 1;
-    hipLaunchKernelGGL(k_chunk_counts, dim3(blocks_for(ncb)), dim3(kBlock), 0, st, ncb, n, (const int32_t*)w.starts,
-                       w.nchunk);
+    const int64_t nkeys = nk - 1;
+    const int64_t ncb = chunk_groups(nk);
+    hipLaunchKernelGGL(k_chunk_counts, dim3(blocks_for(ncb)), dim3(kBlock), 0, st, ncb, nkeys, n,
+                       (const int32_t*)w.starts, w.nchunk);
     FSI_TRY_LAUNCH("mpm_sort chunk counts");
     if (int rc = scan_exclusive(ncb, w.nchunk, w.chunk_off, w.blocksum, st)) return rc;
-    hipLaunchKernelGGL(k_chunk_write, dim3(blocks_for(ncb)), dim3(kBlock), 0, st, ncb, n, (const int32_t*)w.starts,
-                       (const int32_t*)w.nchunk, (const int32_t*)w.chunk_off, max_chunks, chunks, chunk_count);
+    hipLaunchKernelGGL(k_chunk_write, dim3(blocks_for(ncb)), dim3(kBlock), 0, st, ncb, nkeys, n,
+                       (const int32_t*)w.starts, (const int32_t*)w.nchunk, (const int32_t*)w.chunk_off, max_chunks,
+                       chunks, chunk_count);
     FSI_TRY_LAUNCH("mpm_sort chunk write");
   }
   return 0;
@@ -396,7 +420,7 @@ int fsi_mpm_sort(const fsi_mpm_params* p, const fsi_mpm_buffers* src, const fsi_
               (long long)need);
   if (chunks) {
     FSI_REQUIRE(chunk_count, "mpm_sort: chunks without chunk_count");
-    const int64_t ncb = (key_space(make_geom(p)) - 1) / 64 + 1;
+    const int64_t ncb = chunk_groups(key_space(make_geom(p)));
     const int64_t bound = (int64_t)p->num_particles / kBlock + std::min<int64_t>(ncb, p->num_particles) + 1;
     FSI_REQUIRE(max_chunks >= bound, "mpm_sort: room for %d chunks, up to %lld needed", max_chunks,
                 (long long)bound);

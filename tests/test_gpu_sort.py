@@ -17,7 +17,8 @@ pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("gpu_available")]
 def host_keys(pos, active, n3, dx=1.0):
     """Blocked cell keys of fsi_mpm_sort restated on the host: float32
     x * inv_dx, then - 0.5, truncated; clamped to [-1, n]; shifted by one;
-    4x4x4 blocks; inactive particles take the last key."""
+    4x4x4 blocks, row-major with an even number of blocks in z; inactive
+    particles take the last key."""
     inv = np.float32(1.0 / dx)
     gp = (pos.astype(np.float32) * inv).astype(np.float32)
     s = (gp - np.float32(0.5)).astype(np.float32)
@@ -25,6 +26,7 @@ def host_keys(pos, active, n3, dx=1.0):
     n3 = np.asarray(n3)
     c = np.clip(base, -1, n3) + 1
     pb = (n3 + 2 + 3) // 4
+    pb[2] += pb[2] % 2  # an even number of z blocks
     blk = ((c[:, 0] >> 2) * pb[1] + (c[:, 1] >> 2)) * pb[2] + (c[:, 2] >> 2)
     key = blk * 64 + ((c[:, 0] & 3) * 4 + (c[:, 1] & 3)) * 4 + (c[:, 2] & 3)
     nkeys = int(pb.prod()) * 64
