@@ -344,6 +344,10 @@ __global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf
 // ds_add_f32 is processed lane by lane (measured ~20x slower per wave
 // instruction), and the FP64 partial sums are rounded to FP32 once per
 // workgroup at the flush.
+// Grids of at most this many workgroups run the gathers in one batch (see
+// k_p2g): 2 resident workgroups per CU x 256 CUs.
+constexpr int kWideGridLimit = 512;
+
 struct Tile {
   using acc = double;
   static constexpr int kValues = 8;
@@ -364,11 +368,71 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+// ----------------------------------------------- workgroup node box ----
+
+// Bounding box of the base cells of the workgroup's active particles, grown
+// by the 3-node stencil: nodes [x0, x0+tx) x [y0, y0+ty) x [z0, z0+tz). Every
+// thread must call it (it synchronises the workgroup).
+struct NodeBox {
+  int x0, y0, z0, tx, ty, tz;
+  long long nodes;  // 0 when the workgroup has no active particle
+  __device__ int index(int ni, int nj, int nk) const { return ((ni - x0) * ty + (nj - y0)) * tz + (nk - z0); }
+  __device__ void coords(int t, int& ni, int& nj, int& nk) const {
+    nk = z0 + t % tz;
+    nj = y0 + (t / tz) % ty;
+    ni = x0 + t / (tz * ty);
+  }
+};
+
+__device__ NodeBox workgroup_box(bool act, const int base[3], int (*red)[kBlock / 64]) {
+  constexpr int kBig = 1 << 30;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int mn[3], mx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = wave_min(act ? base[a] : kBig);
+    mx[a] = wave_max(act ? base[a] : -kBig);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      red[a][wid] = mn[a];
+      red[3 + a][wid] = mx[a];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = red[a][0];
+    mx[a] = red[3 + a][0];
+#pragma unroll
+    for (int wv = 1; wv < kBlock / 64; ++wv) {
+      mn[a] = min(mn[a], red[a][wv]);
+      mx[a] = max(mx[a], red[3 + a][wv]);
+    }
+  }
+  NodeBox b;
+  const bool any = mn[0] <= mx[0];
+  b.x0 = mn[0];
+  b.y0 = mn[1];
+  b.z0 = mn[2];
+  // extents clamped so that the product cannot overflow for far-apart cells
+  b.tx = any ? (int)min((long long)mx[0] - mn[0] + 3, (long long)kBig) : 0;
+  b.ty = any ? (int)min((long long)mx[1] - mn[1] + 3, (long long)kBig) : 0;
+  b.tz = any ? (int)min((long long)mx[2] - mn[2] + 3, (long long)kBig) : 0;
+  b.nodes = (long long)b.tx * b.ty * b.tz;
+  return b;
+}
+
+
 // fsi/coupling.py:332-406 (COUPLE) fused with fsi/mpm3d.py:234-266 (P2G).
 // The grid and coupling accumulations go through a per-workgroup LDS tile
 // (LDS float atomics) and are flushed with one global atomic per non-zero
 // tile entry; the flush skips nodes outside the lattice, which receive nothing.
-template <typename R, bool COUPLE, bool P2G>
+// WIDE: gather all 27 nodes in one batch (one memory round trip, more
+// registers: for grids that fit on the chip at once); otherwise one i-plane
+// of 9 nodes per batch (fewer registers, more wavefronts per SIMD).
+template <typename R, bool COUPLE, bool P2G, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
                                                 R* __restrict__ v, const R* __restrict__ C,
                                                 R* __restrict__ Fd, const R* __restrict__ mass,
@@ -383,6 +447,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
                                                 int32_t* __restrict__ zero_ctr,
                                                 const int32_t* __restrict__ chunks) {
   __shared__ Tile::acc tile[Tile::kNodes * Tile::kValues];
+  // coupling: lattice velocity and a fluid flag per node of the box
+  __shared__ R ctile[COUPLE ? Tile::kNodes : 1][4];
   // batched substeps: the G2P-pass counters of this substep start at zero
   if (zero_ctr && blockIdx.x == 0 && threadIdx.x < 2) zero_ctr[threadIdx.x] = 0;
   __shared__ int red[6][kBlock / 64];
@@ -473,6 +539,20 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
       atomicAdd(cforce + 4 * (node) + ((slot) - 4), (val)); \
   } while (0)
   for (int t = threadIdx.x; t < tvals; t += blockDim.x) tile[t] = Tile::acc(0);
+  if constexpr (COUPLE) {
+    // stage the box's lattice velocity and fluid flag (in the lattice and
+    // not solid) once per node; the 27-node gathers then read LDS
+    for (int t = threadIdx.x; t < tn_all; t += blockDim.x) {
+      const int ni = x0 + t / (tzi * tyi), nj = y0 + (t / tzi) % tyi, nk = z0 + t % tzi;
+      bool in;
+      const int64_t node = gather_node(m.nx, m.ny, m.nz, ni, nj, nk, in);
+      const bool ok = in && solid[node] == 0;
+      ctile[t][0] = ok ? lbm_vel[node] : R(0);
+      ctile[t][1] = ok ? lbm_vel[N + node] : R(0);
+      ctile[t][2] = ok ? lbm_vel[2 * N + node] : R(0);
+      ctile[t][3] = ok ? R(1) : R(0);
+    }
+  }
   __syncthreads();
 
   if constexpr (COUPLE) {
@@ -482,42 +562,66 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
     }
     if (act) {
       // support of the valid (in-lattice, fluid) nodes and the interpolated
-      // lattice velocity, in i, j, k order; the nodes are gathered one
-      // i-plane (9 nodes, all loads in flight together) at a time
+      // lattice velocity, in i, j, k order; the nodes are gathered in batches
+      // of NB i-planes, all loads of a batch in flight together
       R wsum = R(0), u[3] = {R(0), R(0), R(0)};
       uint32_t valid = 0;  // bit q = (i*3+j)*3+k
+      if (use_lds) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const R* c = ctile[((base[0] + i - x0) * tyi + (base[1] + j - y0)) * tzi + (base[2] + k - z0)];
+              const bool ok = c[3] != R(0);
+              valid |= (uint32_t)ok << ((i * 3 + j) * 3 + k);
+              const R wt = w[i][0] * w[j][1] * w[k][2];
+              wsum += ok ? wt : R(0);
+              u[0] += ok ? wt * c[0] : R(0);
+              u[1] += ok ? wt * c[1] : R(0);
+              u[2] += ok ? wt * c[2] : R(0);
+            }
+      }
+      constexpr int NB = WIDE ? 3 : 1;  // i-planes per gather batch (global path)
 #pragma unroll 1
-      for (int i = 0; i < 3; ++i) {
-        const R wi = pick3(i, w[0][0], w[1][0], w[2][0]);
-        int32_t sn[9];
-        R un[9][3];
+      for (int i0 = use_lds ? 3 : 0; i0 < 3; i0 += NB) {
+        int32_t sn[NB * 9];
+        R un[NB * 9][3];
         uint32_t inmask = 0;
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+        for (int ii = 0; ii < NB; ++ii)
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            bool in;
-            const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
-            const int q = j * 3 + k;
-            sn[q] = solid[node];
-            un[q][0] = lbm_vel[node];
-            un[q][1] = lbm_vel[N + node];
-            un[q][2] = lbm_vel[2 * N + node];
-            inmask |= (uint32_t)in << q;
-          }
+          for (int j = 0; j < 3; ++j)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+              bool in;
+              const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i0 + ii, base[1] + j, base[2] + k, in);
+              const int q = (ii * 3 + j) * 3 + k;
+              sn[q] = solid[node];
+              un[q][0] = lbm_vel[node];
+              un[q][1] = lbm_vel[N + node];
+              un[q][2] = lbm_vel[2 * N + node];
+              inmask |= (uint32_t)in << q;
+            }
 #pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            const int q = j * 3 + k;
-            const bool ok = ((inmask >> q) & 1u) && sn[q] == 0;
-            valid |= (uint32_t)ok << (i * 9 + q);
-            const R wt = wi * w[j][1] * w[k][2];
-            wsum += ok ? wt : R(0);
-            u[0] += ok ? wt * un[q][0] : R(0);
-            u[1] += ok ? wt * un[q][1] : R(0);
-            u[2] += ok ? wt * un[q][2] : R(0);
-          }
+        for (int ii = 0; ii < NB; ++ii) {
+          const int i = i0 + ii;
+          const R wi = pick3(i, w[0][0], w[1][0], w[2][0]);
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const int q = (ii * 3 + j) * 3 + k;
+              const bool ok = ((inmask >> q) & 1u) && sn[q] == 0;
+              valid |= (uint32_t)ok << (i * 9 + j * 3 + k);
+              const R wt = wi * w[j][1] * w[k][2];
+              wsum += ok ? wt : R(0);
+              u[0] += ok ? wt * un[q][0] : R(0);
+              u[1] += ok ? wt * un[q][1] : R(0);
+              u[2] += ok ? wt * un[q][2] : R(0);
+            }
+        }
       }
       valid_weight[p] = wsum;
       if (wsum > cp.min_w) {
@@ -766,7 +870,11 @@ __device__ __forceinline__ bool contact_test(const MpmK<R>& m, const CplK<R>& cp
 
 // fsi/mpm3d.py:268-328: grid update (per gathered node) + g2p + clamp, and
 // the contact pass of fsi/coupling.py:463-496 when CONTACT
-template <typename R, bool CONTACT>
+// Grid nodes a G2P workgroup stages in LDS (the velocity after the grid
+// update, 3 values per node): a work-list chunk spans at most 360.
+constexpr int kG2PNodes = 512;
+
+template <typename R, bool CONTACT, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
                                                 R* __restrict__ v, R* __restrict__ C,
                                                 const int32_t* __restrict__ active,
@@ -776,68 +884,135 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
                                                 int32_t* __restrict__ contact_mask,
                                                 int32_t* __restrict__ counters,
                                                 uint4* __restrict__ zero_grid, int64_t zero_n16,
-                                                int32_t* __restrict__ zero_ctr) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+                                                int32_t* __restrict__ zero_ctr,
+                                                const int32_t* __restrict__ chunks, int num_chunks) {
+  __shared__ R vtile[kG2PNodes][3];
+  __shared__ int red[6][kBlock / 64];
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   // batched substeps: zero the other grid (read by nobody in this pass) for
   // the next P2G pass, and that pass's counters
-  for (int64_t t = p; t < zero_n16; t += (int64_t)gridDim.x * blockDim.x) zero_grid[t] = make_uint4(0, 0, 0, 0);
-  if (zero_ctr && p < 3) zero_ctr[p] = 0;
-  const bool live = p < m.num;
+  for (int64_t t = gt; t < zero_n16; t += (int64_t)gridDim.x * blockDim.x) zero_grid[t] = make_uint4(0, 0, 0, 0);
+  if (zero_ctr && gt < 3) zero_ctr[gt] = 0;
+  // particle of this lane: the work list of the particle sort (workgroups
+  // beyond it only zero), else slot order
+  int p;
+  bool live;
+  if (chunks) {
+    const bool has = (int)blockIdx.x < num_chunks;
+    p = has ? chunks[2 * blockIdx.x] + threadIdx.x : 0;
+    live = has && (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
+  } else {
+    p = gt;
+    live = p < m.num;
+  }
   const bool act = live && active[p] != 0;
   const int cap = m.cap;
   bool cand = false, damped = false;
+  R xp[3] = {R(0), R(0), R(0)}, fx[3] = {R(0), R(0), R(0)}, w[3][3];
+  int base[3] = {0, 0, 0};
   if (act) {
-    R xp[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) xp[a] = x[a * cap + p];
-    int base[3];
-    R fx[3], w[3][3];
     locate(xp, m.inv_dx, base, fx);
-    bspline(fx, w);
+  }
+  bspline(fx, w);
+
+  // stage the grid velocity of the workgroup's node box in LDS: the grid
+  // update (fsi/mpm3d.py:268-288) runs once per node instead of once per
+  // particle and node, and the 27-node gathers read LDS
+  const NodeBox box = workgroup_box(act, base, red);
+  const bool staged = box.nodes > 0 && box.nodes <= kG2PNodes;
+  if (staged) {
+    for (int t = threadIdx.x; t < (int)box.nodes; t += blockDim.x) {
+      int ni, nj, nk;
+      box.coords(t, ni, nj, nk);
+      bool in;
+      const int64_t node = gather_node(m.nx, m.ny, m.nz, ni, nj, nk, in);
+      const R* gp = grid + 4 * node;
+      const R g4[4] = {gp[0], gp[1], gp[2], gp[3]};
+      R gv[3];
+      grid_velocity(m, g4, ni, nj, nk, gv);
+      // out-of-lattice nodes contribute exact zeros
+#pragma unroll
+      for (int a = 0; a < 3; ++a) vtile[t][a] = in ? gv[a] : R(0);
+    }
+    __syncthreads();
+  }
+
+  if (act) {
     R nv[3] = {R(0), R(0), R(0)};
     R nc[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
-    // nodes are gathered one i-plane (9 nodes) at a time: the 9 loads of a
-    // plane are in flight together, and the registers they need leave room
-    // for more wavefronts per SIMD than a 27-node batch. Sums keep the
-    // i, j, k order.
+    // one node's contribution, in the reference's i, j, k order
+    auto accumulate = [&](int i, int j, int k, const R gv[3]) {
+      const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
+      const R wt = w[i][0] * w[j][1] * w[k][2];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) nv[a] += wt * gv[a];
+      const R s = m.four_inv_dx * wt;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
+    };
+    if (staged) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const R* tv = vtile[box.index(base[0] + i, base[1] + j, base[2] + k)];
+            const R gv[3] = {tv[0], tv[1], tv[2]};
+            accumulate(i, j, k, gv);
+          }
+    } else {
+      // box too large for LDS (particles far apart): gather from the grid in
+      // batches of NB i-planes, all loads of a batch in flight together
+      constexpr int NB = WIDE ? 3 : 1;
 #pragma unroll 1
-    for (int i = 0; i < 3; ++i) {
-      const R wi = pick3(i, w[0][0], w[1][0], w[2][0]);
-      R gn[9][4];
-      uint32_t inmask = 0;
+      for (int i0 = 0; i0 < 3; i0 += NB) {
+        R gn[NB * 9][4];
+        uint32_t inmask = 0;
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
+        for (int ii = 0; ii < NB; ++ii)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          bool in;
-          const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
-          const R* gp = grid + 4 * node;
-          const int q = j * 3 + k;
+          for (int j = 0; j < 3; ++j)
 #pragma unroll
-          for (int a = 0; a < 4; ++a) gn[q][a] = gp[a];
-          inmask |= (uint32_t)in << q;
-        }
+            for (int k = 0; k < 3; ++k) {
+              bool in;
+              const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i0 + ii, base[1] + j, base[2] + k, in);
+              const R* gp = grid + 4 * node;
+              const int q = (ii * 3 + j) * 3 + k;
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
+              for (int a = 0; a < 4; ++a) gn[q][a] = gp[a];
+              inmask |= (uint32_t)in << q;
+            }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const int q = j * 3 + k;
-          const bool in = (inmask >> q) & 1u;
-          R gv[3];
-          grid_velocity(m, gn[q], base[0] + i, base[1] + j, base[2] + k, gv);
-          // out-of-lattice nodes contribute exact zeros
+        for (int ii = 0; ii < NB; ++ii)
 #pragma unroll
-          for (int a = 0; a < 3; ++a) gv[a] = in ? gv[a] : R(0);
-          const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
-          const R wt = wi * w[j][1] * w[k][2];
+          for (int j = 0; j < 3; ++j)
 #pragma unroll
-          for (int a = 0; a < 3; ++a) nv[a] += wt * gv[a];
-          const R s = m.four_inv_dx * wt;
+            for (int k = 0; k < 3; ++k) {
+              const int i = i0 + ii;
+              const int q = (ii * 3 + j) * 3 + k;
+              R gv[3];
+              grid_velocity(m, gn[q], base[0] + i, base[1] + j, base[2] + k, gv);
+              const bool in = (inmask >> q) & 1u;
 #pragma unroll
-          for (int r = 0; r < 3; ++r)
+              for (int a = 0; a < 3; ++a) gv[a] = in ? gv[a] : R(0);
+              // w[i][0] with i dynamic when NB == 1
+              const R wi = pick3(i, w[0][0], w[1][0], w[2][0]);
+              const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
+              const R wt = wi * w[j][1] * w[k][2];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
-        }
+              for (int a = 0; a < 3; ++a) nv[a] += wt * gv[a];
+              const R s = m.four_inv_dx * wt;
+#pragma unroll
+              for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
+            }
+      }
     }
 #pragma unroll
     for (int r = 0; r < 9; ++r) C[r * cap + p] = m.apic ? nc[r] : R(0);
@@ -938,14 +1113,26 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
       cb ? (const R*)cb->lbm_vel : nullptr, cb ? (R*)cb->cforce : nullptr,                      \
       cb ? (R*)cb->valid_weight : nullptr, cb ? cb->coupling_mask : nullptr,                    \
       cb ? cb->counters : nullptr, clear_force, zero_ctr, chunks
+  // a grid that fits on the chip at once (2 workgroups per CU with the wide
+  // gathers' registers) takes the one-batch gathers: its time is one
+  // workgroup's latency
+  const bool wide = grid.x <= (unsigned)kWideGridLimit;
+#define FSI_P2G_LAUNCH(C, P)                                                              \
+  do {                                                                                    \
+    if (wide)                                                                             \
+      hipLaunchKernelGGL((k_p2g<R, C, P, true>), grid, block, 0, st, FSI_P2G_ARGS);       \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_p2g<R, C, P, false>), grid, block, 0, st, FSI_P2G_ARGS);      \
+  } while (0)
   if (couple && do_p2g)
-    hipLaunchKernelGGL((k_p2g<R, true, true>), grid, block, 0, st, FSI_P2G_ARGS);
+    FSI_P2G_LAUNCH(true, true);
   else if (couple)
-    hipLaunchKernelGGL((k_p2g<R, true, false>), grid, block, 0, st, FSI_P2G_ARGS);
+    FSI_P2G_LAUNCH(true, false);
   else if (do_p2g)
-    hipLaunchKernelGGL((k_p2g<R, false, true>), grid, block, 0, st, FSI_P2G_ARGS);
+    FSI_P2G_LAUNCH(false, true);
   else
-    hipLaunchKernelGGL((k_p2g<R, false, false>), grid, block, 0, st, FSI_P2G_ARGS);
+    FSI_P2G_LAUNCH(false, false);
+#undef FSI_P2G_LAUNCH
 #undef FSI_P2G_ARGS
   FSI_TRY_LAUNCH("mpm_p2g");
   return 0;
@@ -966,18 +1153,29 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   // 1024 workgroups; particle workgroups beyond that also take part
   const int64_t n16 = zero_grid ? (4LL * p->nx * p->ny * p->nz * (int64_t)sizeof(R)) / 16 : 0;
   const int64_t zero_blocks = n16 ? std::min<int64_t>(blocks_for(n16), 1024) : 0;
-  const int64_t blocks = std::max<int64_t>(m.num ? blocks_for(m.num) : 0, std::max<int64_t>(zero_blocks, zero_ctr ? 1 : 0));
+  // with a work list from the particle sort, one workgroup per chunk (as P2G)
+  const int32_t* chunks = (b->chunks && p->num_chunks > 0) ? b->chunks : nullptr;
+  const int num_chunks = chunks ? p->num_chunks : 0;
+  const int64_t particle_blocks = m.num ? (chunks ? (int64_t)num_chunks : (int64_t)blocks_for(m.num)) : 0;
+  const int64_t blocks = std::max<int64_t>(particle_blocks, std::max<int64_t>(zero_blocks, zero_ctr ? 1 : 0));
   if (blocks == 0) return 0;
-  if (contact)
-    hipLaunchKernelGGL((k_g2p<R, true>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, cp,
-                       (R*)b->x, (R*)b->v, (R*)b->C, b->active, grid_src, cb->solid,
-                       (const R*)cb->cforce, cb->contact_mask, cb->counters, (uint4*)zero_grid, n16,
-                       zero_ctr);
+  // particle workgroups that fit on the chip at once take the one-batch
+  // gathers (as for P2G)
+  const bool wide = particle_blocks <= kWideGridLimit;
+#define FSI_G2P_ARGS(CONTACT_ON)                                                                   \
+  m, cp, (R*)b->x, (R*)b->v, (R*)b->C, b->active, grid_src, CONTACT_ON ? cb->solid : nullptr,      \
+      CONTACT_ON ? (const R*)cb->cforce : nullptr, CONTACT_ON ? cb->contact_mask : nullptr,        \
+      CONTACT_ON ? cb->counters : nullptr, (uint4*)zero_grid, n16, zero_ctr, chunks, num_chunks
+  const dim3 g2p_grid((unsigned)blocks), g2p_block(kBlock);
+  if (contact && wide)
+    hipLaunchKernelGGL((k_g2p<R, true, true>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
+  else if (contact)
+    hipLaunchKernelGGL((k_g2p<R, true, false>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
+  else if (wide)
+    hipLaunchKernelGGL((k_g2p<R, false, true>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(false));
   else
-    hipLaunchKernelGGL((k_g2p<R, false>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, cp,
-                       (R*)b->x, (R*)b->v, (R*)b->C, b->active, grid_src,
-                       (const int32_t*)nullptr, (const R*)nullptr, (int32_t*)nullptr,
-                       (int32_t*)nullptr, (uint4*)zero_grid, n16, zero_ctr);
+    hipLaunchKernelGGL((k_g2p<R, false, false>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(false));
+#undef FSI_G2P_ARGS
   FSI_TRY_LAUNCH("mpm_g2p");
   return 0;
 }
