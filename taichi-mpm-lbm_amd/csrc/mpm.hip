@@ -125,12 +125,12 @@ __device__ __forceinline__ int64_t gather_node(int nx, int ny, int nz, int i, in
   return in ? ((int64_t)i * ny + j) * nz + k : 0;
 }
 
-// stencil offset q = (i*3+j)*3+k of the 27 nodes, decoded without division
-__device__ __forceinline__ void stencil_ijk(int q, int& i, int& j, int& k) {
-  i = (q >= 9) + (q >= 18);
-  const int r = q - 9 * i;
-  j = (r >= 3) + (r >= 6);
-  k = r - 3 * j;
+// out[s] = (a0, a1, a2)[(s + r) % 3] for r in {0, 1, 2}, with selects
+template <typename T>
+__device__ __forceinline__ void rotate3(int r, T a0, T a1, T a2, T out[3]) {
+  out[0] = r == 0 ? a0 : (r == 1 ? a1 : a2);
+  out[1] = r == 0 ? a1 : (r == 1 ? a2 : a0);
+  out[2] = r == 0 ? a2 : (r == 1 ? a0 : a1);
 }
 
 template <typename R> __device__ __forceinline__ R pick3(int i, R a, R b, R c) {
@@ -327,6 +327,69 @@ template <typename R> __device__ __forceinline__ R det3(const R m[9]) {
 
 // ------------------------------------------------------- P2G pass ----
 
+// F <- (I + dt C) F (fsi/mpm3d.py:243)
+template <typename R>
+__device__ __forceinline__ void deformation_update(R dt, const R Cm[9], const R Fo[9], R Fn[9]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int s2 = 0; s2 < 3; ++s2) {
+      R acc = R(0);
+#pragma unroll
+      for (int t = 0; t < 3; ++t) acc += ((r == t ? R(1) : R(0)) + dt * Cm[r * 3 + t]) * Fo[t * 3 + s2];
+      Fn[r * 3 + s2] = acc;
+    }
+}
+
+// Affine momentum matrix and the particle's momentum (fsi/mpm3d.py:244-262):
+// P = 2 mu (F - R) F^T + lam J (J - 1) I, aff = -dt vol 4/dx^2 P + m C,
+// mom0 = m v + dt f (the coupling force of this substep)
+template <typename R>
+__device__ __forceinline__ void affine_momentum(const MpmK<R>& m, const R Fn[9], const R Rot[9], const R Cm[9],
+                                                const R vp[3], const R pf[3], R mp, R vp_vol, R aff[9],
+                                                R mom0[3]) {
+  const R J = det3(Fn);
+  const R scale = -m.dt * vp_vol * R(4) * m.inv_dx * m.inv_dx;
+  const R lj = m.lam * J * (J - R(1));
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int s2 = 0; s2 < 3; ++s2) {
+      R st = R(0);
+#pragma unroll
+      for (int t = 0; t < 3; ++t) st += (m.two_mu * (Fn[r * 3 + t] - Rot[r * 3 + t])) * Fn[s2 * 3 + t];
+      if (r == s2) st += lj;
+      aff[r * 3 + s2] = scale * st + mp * Cm[r * 3 + s2];
+    }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) mom0[a] = mp * vp[a] + m.dt * pf[a];
+}
+
+// P2G scatter of one particle straight to the global grid (in-lattice nodes
+// only), for workgroups whose particles do not fit an LDS tile
+template <typename R>
+__device__ __forceinline__ void scatter_global(const MpmK<R>& m, R* __restrict__ grid, const int base[3],
+                                               const R fx[3], const R w[3][3], const R mom0[3], const R aff[9],
+                                               R mp) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+        if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+        const R dp[3] = {(R(i) - fx[0]) * m.dx, (R(j) - fx[1]) * m.dx, (R(k) - fx[2]) * m.dx};
+        const R wt = w[i][0] * w[j][1] * w[k][2];
+        const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          atomicAdd(grid + 4 * node + a,
+                    wt * (mom0[a] + (aff[a * 3 + 0] * dp[0] + aff[a * 3 + 1] * dp[1] + aff[a * 3 + 2] * dp[2])));
+        atomicAdd(grid + 4 * node + 3, wt * mp);
+      }
+}
+
 // grid clear (fsi/mpm3d.py:205-209) and, when given, the per-substep
 // particle-diagnostic counters (fsi/coupling.py:291-297)
 template <typename R>
@@ -337,24 +400,25 @@ __global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf
   if (counters && t < 5) counters[t] = 0;
 }
 
-// LDS tile of one workgroup: 8 values per node (grid momentum xyz + mass,
-// coupling force density xyz + phi), 64 KB, so two workgroups fit a CU.
-// Per-workgroup P2G accumulation tile in LDS. The words are FP64 for both
-// precisions: on gfx950 ds_add_f64 issues at the integer-atomic rate, while
-// ds_add_f32 is processed lane by lane (measured ~20x slower per wave
-// instruction), and the FP64 partial sums are rounded to FP32 once per
-// workgroup at the flush.
 // Grids of at most this many workgroups run the gathers in one batch (see
 // k_p2g): 2 resident workgroups per CU x 256 CUs.
 constexpr int kWideGridLimit = 512;
 
+// Per-workgroup P2G accumulation tile in LDS: 8 values per node (grid
+// momentum xyz + mass, coupling force density xyz + phi), value-major with a
+// fixed stride of kNodes words, so the value index is an immediate offset of
+// the LDS atomic. The words are FP64 for both precisions: on gfx950
+// ds_add_f64 issues at the integer-atomic rate, while ds_add_f32 is processed
+// lane by lane (measured ~20x slower per wave instruction), and the FP64
+// partial sums are rounded to FP32 once per workgroup at the flush.
 struct Tile {
   using acc = double;
   static constexpr int kValues = 8;
-  // 32 KB: a work-list chunk (one 4x4x4 cell block) spans at most ~6^3
-  // nodes plus drift since the last sort; 32 KB per workgroup leaves LDS for
-  // four workgroups per CU
-  static constexpr int kNodes = (32 * 1024) / (kValues * (int)sizeof(acc));
+  // a work-list chunk (two z-adjacent 4x4x4 cell blocks) spans 6x6x10 = 360
+  // nodes plus drift since the last sort; 496 nodes keep the tile (31 KB)
+  // and the coupling's staged lattice velocity (7.75 KB, float) under 40 KB,
+  // so four workgroups fit the 160 KB of a CU
+  static constexpr int kNodes = 496;
 };
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -446,9 +510,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
                                                 int32_t* __restrict__ counters, int clear_force,
                                                 int32_t* __restrict__ zero_ctr,
                                                 const int32_t* __restrict__ chunks) {
-  __shared__ Tile::acc tile[Tile::kNodes * Tile::kValues];
-  // coupling: lattice velocity and a fluid flag per node of the box
-  __shared__ R ctile[COUPLE ? Tile::kNodes : 1][4];
+  using A = Tile::acc;
+  constexpr int kN = Tile::kNodes;
+  __shared__ A tile[kN * Tile::kValues];
+  // coupling: lattice velocity and a fluid flag per node of the box (zero
+  // velocity and flag on solid and out-of-lattice nodes)
+  __shared__ __align__(16) R ctile[COUPLE ? kN : 1][4];
   // batched substeps: the G2P-pass counters of this substep start at zero
   if (zero_ctr && blockIdx.x == 0 && threadIdx.x < 2) zero_ctr[threadIdx.x] = 0;
   __shared__ int red[6][kBlock / 64];
@@ -520,17 +587,13 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
   const long long tnodes = tx * ty * tz;
   // scatter target of the workgroup: its LDS tile when the cells of its
   // particles fit, otherwise the global grid/coupling buffers directly
-  const bool use_lds = any && tnodes <= Tile::kNodes;
+  const bool use_lds = any && tnodes <= kN;
   const int x0 = mn[0], y0 = mn[1], z0 = mn[2];
   const int tyi = use_lds ? (int)ty : 0, tzi = use_lds ? (int)tz : 0;
-  const int tvals = use_lds ? (int)tnodes * Tile::kValues : 0;
-  // LDS tile, slot-major (tile[slot * tnodes + node]) so neighbouring nodes
-  // of one slot fall in different banks; it spans base..base+2 of every
-  // particle of the workgroup, so the scatter needs no per-node bounds test
-  // (out-of-lattice entries are dropped at the flush)
+  // the tile spans base..base+2 of every particle of the workgroup, so the
+  // scatter needs no per-node bounds test; out-of-lattice entries, and the
+  // coupling values of solid nodes, are dropped at the flush
   const int tn_all = use_lds ? (int)tnodes : 0;
-#define FSI_TILE(ni, nj, nk, slot) \
-  tile[(slot) * tn_all + (((ni) - x0) * tyi + ((nj) - y0)) * tzi + ((nk) - z0)]
 #define FSI_GLOBAL(node, slot, val)                      \
   do {                                                   \
     if ((slot) < 4)                                      \
@@ -538,22 +601,43 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
     else                                                 \
       atomicAdd(cforce + 4 * (node) + ((slot) - 4), (val)); \
   } while (0)
-  for (int t = threadIdx.x; t < tvals; t += blockDim.x) tile[t] = Tile::acc(0);
-  if constexpr (COUPLE) {
-    // stage the box's lattice velocity and fluid flag (in the lattice and
-    // not solid) once per node; the 27-node gathers then read LDS
-    for (int t = threadIdx.x; t < tn_all; t += blockDim.x) {
+  for (int t = threadIdx.x; t < tn_all; t += blockDim.x) {
+#pragma unroll
+    for (int s = 0; s < (COUPLE ? Tile::kValues : 4); ++s) tile[s * kN + t] = A(0);
+    if constexpr (COUPLE) {
+      // stage the box's lattice velocity and fluid flag (in the lattice and
+      // not solid) once per node; the 27-node gathers then read LDS
       const int ni = x0 + t / (tzi * tyi), nj = y0 + (t / tzi) % tyi, nk = z0 + t % tzi;
       bool in;
       const int64_t node = gather_node(m.nx, m.ny, m.nz, ni, nj, nk, in);
-      const bool ok = in && solid[node] == 0;
-      ctile[t][0] = ok ? lbm_vel[node] : R(0);
-      ctile[t][1] = ok ? lbm_vel[N + node] : R(0);
-      ctile[t][2] = ok ? lbm_vel[2 * N + node] : R(0);
+      const int32_t sd = solid[node];
+      const R u0 = lbm_vel[node], u1 = lbm_vel[N + node], u2 = lbm_vel[2 * N + node];
+      const bool ok = in && sd == 0;
+      ctile[t][0] = ok ? u0 : R(0);
+      ctile[t][1] = ok ? u1 : R(0);
+      ctile[t][2] = ok ? u2 : R(0);
       ctile[t][3] = ok ? R(1) : R(0);
     }
   }
   __syncthreads();
+  // lane-rotated node order of the LDS scatters: lane walks j and k from
+  // offsets (lane % 3, lane / 3 % 3), so up to 9 particles of one cell in
+  // consecutive lanes add to 9 different nodes on every instruction (LDS
+  // atomics to one address serialise); the order is static, the rotated
+  // weights, offsets and tile rows are formed once per particle
+  const int rj = lane % 3, rk = (lane / 3) % 3;
+  int jo[3], ko[3];
+  rotate3(rj, 0, 1, 2, jo);
+  rotate3(rk, 0, 1, 2, ko);
+  // tile word of node (base + (i, jo[jj], ko[kk])): row[jj] + i*tyz + ko[kk]
+  const int tyz = tyi * tzi;
+  const int b0 = ((base[0] - x0) * tyi + (base[1] - y0)) * tzi + (base[2] - z0);
+  int row[3];
+#pragma unroll
+  for (int jj = 0; jj < 3; ++jj) row[jj] = b0 + jo[jj] * tzi;
+  A wjr[3], wkr[3];
+  rotate3<A>(rj, w[0][1], w[1][1], w[2][1], wjr);
+  rotate3<A>(rk, w[0][2], w[1][2], w[2][2], wkr);
 
   if constexpr (COUPLE) {
     if (live && !act) {
@@ -565,22 +649,23 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
       // lattice velocity, in i, j, k order; the nodes are gathered in batches
       // of NB i-planes, all loads of a batch in flight together
       R wsum = R(0), u[3] = {R(0), R(0), R(0)};
-      uint32_t valid = 0;  // bit q = (i*3+j)*3+k
+      uint32_t valid = 0;  // global path: bit q = (i*3+j)*3+k
       if (use_lds) {
+        // staged values are zero on invalid nodes and the flag is 0 or 1, so
+        // the products are the masked sums without selects
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-              const R* c = ctile[((base[0] + i - x0) * tyi + (base[1] + j - y0)) * tzi + (base[2] + k - z0)];
-              const bool ok = c[3] != R(0);
-              valid |= (uint32_t)ok << ((i * 3 + j) * 3 + k);
+              const R* c = ctile[b0 + i * tyz + j * tzi + k];
+              const R c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
               const R wt = w[i][0] * w[j][1] * w[k][2];
-              wsum += ok ? wt : R(0);
-              u[0] += ok ? wt * c[0] : R(0);
-              u[1] += ok ? wt * c[1] : R(0);
-              u[2] += ok ? wt * c[2] : R(0);
+              wsum += wt * c3;
+              u[0] += wt * c0;
+              u[1] += wt * c1;
+              u[2] += wt * c2;
             }
       }
       constexpr int NB = WIDE ? 3 : 1;  // i-planes per gather batch (global path)
@@ -655,25 +740,33 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
         const R phi = vp_vol / cp.cell_volume;
         const R rf[3] = {-pf[0] * fscale, -pf[1] * fscale, -pf[2] * fscale};
         if (use_lds) {
-          // each lane walks the 27 nodes from its own starting offset, so
-          // lanes sharing a cell add to different LDS words on each
-          // instruction (same-address LDS atomics serialise); invalid nodes
-          // add exact zeros
-          int q = lane % 27;
-          for (int o = 0; o < 27; ++o) {
-            int i, j, k;
-            stencil_ijk(q, i, j, k);
-            const bool ok = (valid >> q) & 1u;
-            const R wt = pick3(i, w[0][0], w[1][0], w[2][0]) * pick3(j, w[0][1], w[1][1], w[2][1]) *
-                         pick3(k, w[0][2], w[1][2], w[2][2]);
-            const R nv = ok ? wt * inv_wsum : R(0);
-            const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-            atomicAdd(&FSI_TILE(ni, nj, nk, 4), nv * rf[0]);
-            atomicAdd(&FSI_TILE(ni, nj, nk, 5), nv * rf[1]);
-            atomicAdd(&FSI_TILE(ni, nj, nk, 6), nv * rf[2]);
-            if (cp.build) atomicAdd(&FSI_TILE(ni, nj, nk, 7), nv * phi);
-            q = q == 26 ? 0 : q + 1;
-          }
+          // all 27 nodes in the lane-rotated order; the values of invalid
+          // (solid or out-of-lattice) nodes are dropped at the flush, which
+          // leaves the valid nodes' sums unchanged
+          const A iw = A(inv_wsum);
+          const A rfn[4] = {A(rf[0]) * iw, A(rf[1]) * iw, A(rf[2]) * iw, A(phi) * iw};
+          auto spread = [&](auto with_phi) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int jj = 0; jj < 3; ++jj) {
+                const A wij = A(w[i][0]) * wjr[jj];
+                A* tr = tile + row[jj] + i * tyz;
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                  const A wt = wij * wkr[kk];
+                  A* tn = tr + ko[kk];
+                  atomicAdd(tn + 4 * kN, wt * rfn[0]);
+                  atomicAdd(tn + 5 * kN, wt * rfn[1]);
+                  atomicAdd(tn + 6 * kN, wt * rfn[2]);
+                  if constexpr (decltype(with_phi)::value) atomicAdd(tn + 7 * kN, wt * rfn[3]);
+                }
+              }
+          };
+          if (cp.build)
+            spread(std::true_type{});
+          else
+            spread(std::false_type{});
         } else {
 #pragma unroll
           for (int i = 0; i < 3; ++i)
@@ -715,106 +808,128 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
     }
   }
 
+  // FP32 particles whose F needs the FP64 SVD are scattered after the flush,
+  // where little else is live (keeps the kernel at four waves per SIMD)
+  bool deferred = false;
   if constexpr (P2G) {
     if (act) {
-      const R dt = m.dt;
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
         Cm[r] = C[r * cap + p];
         Fo[r] = Fd[r * cap + p];
       }
       R Fn[9];
-      // F <- (I + dt C) F
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int s2 = 0; s2 < 3; ++s2) {
-          R acc = R(0);
-#pragma unroll
-          for (int t = 0; t < 3; ++t) acc += ((r == t ? R(1) : R(0)) + dt * Cm[r * 3 + t]) * Fo[t * 3 + s2];
-          Fn[r * 3 + s2] = acc;
-        }
+      deformation_update(m.dt, Cm, Fo, Fn);
 #pragma unroll
       for (int r = 0; r < 9; ++r) Fd[r * cap + p] = Fn[r];
-      const R J = det3(Fn);
       R Rot[9];
-      rotation_factor<R>(Fn, Rot, J);
-      // P = 2 mu (F - R) F^T + lam J (J - 1) I; affine = -dt vol 4/dx^2 P + m C
-      const R scale = -dt * vp_vol * R(4) * m.inv_dx * m.inv_dx;
-      const R lj = m.lam * J * (J - R(1));
-      R aff[9];
+      bool have_rot = true;
+      if constexpr (sizeof(R) == 8)
+        rotation_core<R>(Fn, Rot, nullptr);
+      else
+        have_rot = polar_newton(Fn, Rot);
+      deferred = !have_rot;
+      if (have_rot) {
+        R aff[9], mom0[3];
+        affine_momentum(m, Fn, Rot, Cm, vp, pf, mp, vp_vol, aff, mom0);
+        if (use_lds) {
+          // lane-rotated order as for the coupling scatter; in FP64, the
+          // accumulator type, the affine momentum is built up separably:
+          // mom0 + aff (dp_i, dp_j, dp_k) per i plane, (i, j) row and node
+          const A dx = A(m.dx);
+          A dpj[3], dpk[3], ak[3];
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+          for (int s = 0; s < 3; ++s) {
+            dpj[s] = (A(jo[s]) - A(fx[1])) * dx;
+            dpk[s] = (A(ko[s]) - A(fx[2])) * dx;
+            ak[s] = A(aff[s * 3 + 2]);
+          }
+          const A mpa = A(mp);
 #pragma unroll
-        for (int s2 = 0; s2 < 3; ++s2) {
-          R st = R(0);
+          for (int i = 0; i < 3; ++i) {
+            const A dpi = (A(i) - A(fx[0])) * dx;
+            A mi[3];
 #pragma unroll
-          for (int t = 0; t < 3; ++t) st += (m.two_mu * (Fn[r * 3 + t] - Rot[r * 3 + t])) * Fn[s2 * 3 + t];
-          if (r == s2) st += lj;
-          aff[r * 3 + s2] = scale * st + mp * Cm[r * 3 + s2];
-        }
-      R mom0[3];
+            for (int a = 0; a < 3; ++a) mi[a] = A(mom0[a]) + A(aff[a * 3 + 0]) * dpi;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) mom0[a] = mp * vp[a] + dt * pf[a];
-      if (use_lds) {
-        // rotated walk over the 27 nodes, as for the coupling scatter above
-        int q = lane % 27;
-        for (int o = 0; o < 27; ++o) {
-          int i, j, k;
-          stencil_ijk(q, i, j, k);
-          const R dp[3] = {(R(i) - fx[0]) * m.dx, (R(j) - fx[1]) * m.dx, (R(k) - fx[2]) * m.dx};
-          const R wt = pick3(i, w[0][0], w[1][0], w[2][0]) * pick3(j, w[0][1], w[1][1], w[2][1]) *
-                       pick3(k, w[0][2], w[1][2], w[2][2]);
-          const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
+            for (int jj = 0; jj < 3; ++jj) {
+              const A wij = A(w[i][0]) * wjr[jj];
+              A mij[3];
 #pragma unroll
-          for (int a = 0; a < 3; ++a)
-            atomicAdd(&FSI_TILE(ni, nj, nk, a),
-                      wt * (mom0[a] + (aff[a * 3 + 0] * dp[0] + aff[a * 3 + 1] * dp[1] + aff[a * 3 + 2] * dp[2])));
-          atomicAdd(&FSI_TILE(ni, nj, nk, 3), wt * mp);
-          q = q == 26 ? 0 : q + 1;
-        }
-      } else {
+              for (int a = 0; a < 3; ++a) mij[a] = mi[a] + A(aff[a * 3 + 1]) * dpj[jj];
+              A* tr = tile + row[jj] + i * tyz;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+              for (int kk = 0; kk < 3; ++kk) {
+                const A wt = wij * wkr[kk];
+                A* tn = tr + ko[kk];
 #pragma unroll
-          for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-              if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
-              const R dp[3] = {(R(i) - fx[0]) * m.dx, (R(j) - fx[1]) * m.dx, (R(k) - fx[2]) * m.dx};
-              const R wt = w[i][0] * w[j][1] * w[k][2];
-              const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
-#pragma unroll
-              for (int a = 0; a < 3; ++a)
-                atomicAdd(grid + 4 * node + a,
-                          wt * (mom0[a] + (aff[a * 3 + 0] * dp[0] + aff[a * 3 + 1] * dp[1] + aff[a * 3 + 2] * dp[2])));
-              atomicAdd(grid + 4 * node + 3, wt * mp);
+                for (int a = 0; a < 3; ++a) atomicAdd(tn + a * kN, wt * (mij[a] + ak[a] * dpk[kk]));
+                atomicAdd(tn + 3 * kN, wt * mpa);
+              }
             }
+          }
+        } else {
+          scatter_global(m, grid, base, fx, w, mom0, aff, mp);
+        }
       }
     }
   }
 
-#undef FSI_TILE
 #undef FSI_GLOBAL
-  // flush the tile: one global atomic per non-zero entry of an in-lattice node
+  // flush the tile: one global atomic per non-zero entry of an in-lattice
+  // node; coupling entries only on fluid nodes
   if (use_lds) {
     __syncthreads();
-    for (int t = threadIdx.x; t < tvals; t += blockDim.x) {
-      const R val = R(tile[t]);
-      if (val == R(0)) continue;
-      const int slot = t / tn_all;
-      const int tn = t - slot * tn_all;
-      const int lk = tn % tzi;
-      const int lj = (tn / tzi) % tyi;
-      const int li = tn / (tzi * tyi);
-      const int ni = x0 + li, nj = y0 + lj, nk = z0 + lk;
+    for (int t = threadIdx.x; t < tn_all; t += blockDim.x) {
+      const int ni = x0 + t / tyz, nj = y0 + (t / tzi) % tyi, nk = z0 + t % tzi;
       if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
       const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
-      if (slot < 4)
-        atomicAdd(grid + 4 * node + slot, val);
-      else
-        atomicAdd(cforce + 4 * node + (slot - 4), val);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const R val = R(tile[s * kN + t]);
+        if (val != R(0)) atomicAdd(grid + 4 * node + s, val);
+      }
+      if constexpr (COUPLE) {
+        if (ctile[t][3] != R(0)) {
+#pragma unroll
+          for (int s = 4; s < Tile::kValues; ++s) {
+            const R val = R(tile[s * kN + t]);
+            if (val != R(0)) atomicAdd(cforce + 4 * node + (s - 4), val);
+          }
+        }
+      }
+    }
+  }
+  if constexpr (P2G && sizeof(R) == 4) {
+    // inverted or ill-conditioned F: reload the particle (F is the updated
+    // one stored above, pforce holds this substep's coupling force) and use
+    // the FP64 Jacobi rotation; global atomics, as on the untiled path
+    if (deferred) {
+      R xr[3], vr[3], fr[3], wr[3][3], Cr[9], Fr[9], pr[3];
+      int br[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        xr[a] = x[a * cap + p];
+        vr[a] = v[a * cap + p];
+        pr[a] = pforce[a * cap + p];
+      }
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        Cr[r] = C[r * cap + p];
+        Fr[r] = Fd[r * cap + p];
+      }
+      const R mr = mass[p], volr = vol[p];
+      locate(xr, m.inv_dx, br, fr);
+      bspline(fr, wr);
+      double Fdd[9], Rdd[9];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) Fdd[r] = double(Fr[r]);
+      rotation_core<double>(Fdd, Rdd, nullptr);
+      R Rot[9], aff[9], mom0[3];
+#pragma unroll
+      for (int r = 0; r < 9; ++r) Rot[r] = R(Rdd[r]);
+      affine_momentum(m, Fr, Rot, Cr, vr, pr, mr, volr, aff, mom0);
+      scatter_global(m, grid, br, fr, wr, mom0, aff, mr);
     }
   }
 }

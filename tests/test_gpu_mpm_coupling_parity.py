@@ -74,6 +74,56 @@ def test_mpm_fp32_within_reference_band(mpm_oracle):
         assert e_gpu <= 2.0 * e_ref + 2e-6, (key, e_gpu, e_ref)
 
 
+def _hard_deformations(n, seed):
+    """Mostly mild F, with a tenth inverted (det < 0) or near singular
+    (Frobenius condition number > 300): the FP32 build computes those
+    particles' rotation with the FP64 Jacobi SVD, after the LDS tile flush."""
+    rng = np.random.default_rng(seed)
+    F = np.eye(3) + 0.05 * rng.standard_normal((n, 3, 3))
+    idx = rng.choice(n, n // 10, replace=False)
+    inv, sing = idx[: len(idx) // 2], idx[len(idx) // 2:]
+    F[inv, 0, :] *= -1.0
+    F[sing, 2, :] = F[sing, 1, :] + 1e-3 * rng.standard_normal((len(sing), 3))
+    kappa = np.linalg.norm(F[sing], axis=(1, 2)) * np.linalg.norm(np.linalg.inv(F[sing]), axis=(1, 2))
+    assert np.all(np.linalg.det(F[inv]) < 0) and np.all(kappa > 300.0)
+    return F
+
+
+@pytest.mark.parametrize("sort", [True, False])
+def test_mpm_fp32_hard_deformations(sort):
+    """Inverted and ill-conditioned particles among ordinary ones: dropped or
+    doubled momentum of the deferred particles would move x/v far outside
+    the band."""
+    import fsi
+
+    cfg, pos, vel = _spinning_cube()
+    n = len(pos)
+    F = _hard_deformations(n, seed=5)
+    outs = {}
+    for p in ("f32", "f64"):
+        o = po.OracleMPM(cfg, p)
+        o.initialize_particles_from_numpy(pos, vel, particle_mass=0.15, particle_volume=0.125)
+        o.F[:n] = F
+        for _ in range(3):
+            o.substep()
+        outs[p] = {"x": o.x[:n].copy(), "v": o.v[:n].copy(), "F": o.F[:n].copy()}
+    s = fsi.MPMSolver3D(cfg)
+    s.initialize_particles_from_numpy(pos, vel, particle_mass=0.15, particle_volume=0.125)
+    import torch
+
+    s.F.view(9, s.max_particles)[:, :n] = torch.from_numpy(F.reshape(n, 9).T.astype(np.float32)).cuda()
+    if sort:
+        s.sort_particles()
+    for _ in range(3):
+        s.substep()
+    got = {"x": s.positions_numpy(), "v": s.velocities_numpy(), "F": s.deformation_gradients_numpy()}
+    for key in ("x", "v", "F"):
+        e_gpu = rel_inf(got[key], outs["f64"][key])
+        e_ref = rel_inf(outs["f32"][key], outs["f64"][key])
+        print(f"hard F sort={sort} {key}: gpu-vs-f64 {e_gpu:.3e} ref32-vs-f64 {e_ref:.3e}")
+        assert e_gpu <= 2.0 * e_ref + 2e-6, (key, e_gpu, e_ref)
+
+
 def _random_deformations(n, seed):
     rng = np.random.default_rng(seed)
     F = np.eye(3) + 0.3 * rng.standard_normal((n, 3, 3))
