@@ -620,22 +620,30 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
     }
   }
   __syncthreads();
-  // lane-rotated node order of the LDS scatters: lane walks j and k from
-  // offsets (lane % 3, lane / 3 % 3), so up to 9 particles of one cell in
-  // consecutive lanes add to 9 different nodes on every instruction (LDS
-  // atomics to one address serialise); the order is static, the rotated
-  // weights, offsets and tile rows are formed once per particle
-  const int rj = lane % 3, rk = (lane / 3) % 3;
-  int jo[3], ko[3];
+  // lane-rotated node order of the LDS scatters: lane walks i, j and k from
+  // offsets (lane / 9 % 3, lane % 3, lane / 3 % 3), so up to 27 particles
+  // in consecutive lanes add to 27 different nodes on every instruction, and
+  // the lanes of neighbouring cells spread over three x planes (LDS atomics
+  // to one address serialise, ~12 clk per extra lane on gfx950); the order
+  // is static, the rotated weights, offsets and tile rows are formed once
+  // per particle
+  const int ri = (lane / 9) % 3, rj = lane % 3, rk = (lane / 3) % 3;
+  int io[3], jo[3], ko[3];
+  rotate3(ri, 0, 1, 2, io);
   rotate3(rj, 0, 1, 2, jo);
   rotate3(rk, 0, 1, 2, ko);
-  // tile word of node (base + (i, jo[jj], ko[kk])): row[jj] + i*tyz + ko[kk]
+  // tile word of node (base + (io[ii], jo[jj], ko[kk])):
+  // plane[ii] + row[jj] + ko[kk]
   const int tyz = tyi * tzi;
   const int b0 = ((base[0] - x0) * tyi + (base[1] - y0)) * tzi + (base[2] - z0);
-  int row[3];
+  int row[3], plane[3];
 #pragma unroll
-  for (int jj = 0; jj < 3; ++jj) row[jj] = b0 + jo[jj] * tzi;
-  A wjr[3], wkr[3];
+  for (int s = 0; s < 3; ++s) {
+    row[s] = b0 + jo[s] * tzi;
+    plane[s] = io[s] * tyz;
+  }
+  A wir[3], wjr[3], wkr[3];
+  rotate3<A>(ri, w[0][0], w[1][0], w[2][0], wir);
   rotate3<A>(rj, w[0][1], w[1][1], w[2][1], wjr);
   rotate3<A>(rk, w[0][2], w[1][2], w[2][2], wkr);
 
@@ -747,11 +755,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
           const A rfn[4] = {A(rf[0]) * iw, A(rf[1]) * iw, A(rf[2]) * iw, A(phi) * iw};
           auto spread = [&](auto with_phi) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int ii = 0; ii < 3; ++ii)
 #pragma unroll
               for (int jj = 0; jj < 3; ++jj) {
-                const A wij = A(w[i][0]) * wjr[jj];
-                A* tr = tile + row[jj] + i * tyz;
+                const A wij = wir[ii] * wjr[jj];
+                A* tr = tile + row[jj] + plane[ii];
 #pragma unroll
                 for (int kk = 0; kk < 3; ++kk) {
                   const A wt = wij * wkr[kk];
@@ -846,18 +854,18 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
           }
           const A mpa = A(mp);
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            const A dpi = (A(i) - A(fx[0])) * dx;
+          for (int ii = 0; ii < 3; ++ii) {
+            const A dpi = (A(io[ii]) - A(fx[0])) * dx;
             A mi[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) mi[a] = A(mom0[a]) + A(aff[a * 3 + 0]) * dpi;
 #pragma unroll
             for (int jj = 0; jj < 3; ++jj) {
-              const A wij = A(w[i][0]) * wjr[jj];
+              const A wij = wir[ii] * wjr[jj];
               A mij[3];
 #pragma unroll
               for (int a = 0; a < 3; ++a) mij[a] = mi[a] + A(aff[a * 3 + 1]) * dpj[jj];
-              A* tr = tile + row[jj] + i * tyz;
+              A* tr = tile + row[jj] + plane[ii];
 #pragma unroll
               for (int kk = 0; kk < 3; ++kk) {
                 const A wt = wij * wkr[kk];
