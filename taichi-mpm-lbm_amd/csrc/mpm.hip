@@ -137,6 +137,10 @@ template <typename R> __device__ __forceinline__ R pick3(int i, R a, R b, R c) {
   return i == 0 ? a : (i == 1 ? b : c);
 }
 
+template <typename R> __device__ __forceinline__ R pick4(int i, R a, R b, R c, R d) {
+  return i < 2 ? (i == 0 ? a : b) : (i == 2 ? c : d);
+}
+
 // ------------------------------------------------------------- SVD ----
 //
 // Rotation factor R = U V^T of the rotation-variant SVD F = U S V^T (U, V
@@ -753,7 +757,20 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
           // leaves the valid nodes' sums unchanged
           const A iw = A(inv_wsum);
           const A rfn[4] = {A(rf[0]) * iw, A(rf[1]) * iw, A(rf[2]) * iw, A(phi) * iw};
+          // the value order is rotated by lane as well (instruction k adds
+          // value (k + r) % NV), so lanes that meet on one node in the same
+          // instruction mostly write different values, i.e. different words
           auto spread = [&](auto with_phi) {
+            constexpr int NV = decltype(with_phi)::value ? 4 : 3;
+            const int r = NV == 4 ? (lane & 3) : (lane >> 2) % 3;
+            A val[NV];
+            int off[NV];
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+              const int c = (k + r) % NV;
+              val[k] = NV == 4 ? pick4(c, rfn[0], rfn[1], rfn[2], rfn[3]) : pick3(c, rfn[0], rfn[1], rfn[2]);
+              off[k] = (4 + c) * kN;
+            }
 #pragma unroll
             for (int ii = 0; ii < 3; ++ii)
 #pragma unroll
@@ -764,10 +781,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
                 for (int kk = 0; kk < 3; ++kk) {
                   const A wt = wij * wkr[kk];
                   A* tn = tr + ko[kk];
-                  atomicAdd(tn + 4 * kN, wt * rfn[0]);
-                  atomicAdd(tn + 5 * kN, wt * rfn[1]);
-                  atomicAdd(tn + 6 * kN, wt * rfn[2]);
-                  if constexpr (decltype(with_phi)::value) atomicAdd(tn + 7 * kN, wt * rfn[3]);
+#pragma unroll
+                  for (int k = 0; k < NV; ++k) atomicAdd(tn + off[k], wt * val[k]);
                 }
               }
           };
@@ -844,35 +859,46 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
           // lane-rotated order as for the coupling scatter; in FP64, the
           // accumulator type, the affine momentum is built up separably:
           // mom0 + aff (dp_i, dp_j, dp_k) per i plane, (i, j) row and node
+          // The four values (momentum xyz, mass) are written in a
+          // lane-rotated order too: instruction k adds value (k + lane) % 4,
+          // with the mass as a fourth component of zero affine part
           const A dx = A(m.dx);
-          A dpj[3], dpk[3], ak[3];
+          A dpj[3], dpk[3];
 #pragma unroll
           for (int s = 0; s < 3; ++s) {
             dpj[s] = (A(jo[s]) - A(fx[1])) * dx;
             dpk[s] = (A(ko[s]) - A(fx[2])) * dx;
-            ak[s] = A(aff[s * 3 + 2]);
           }
-          const A mpa = A(mp);
+          A cm[4], cx[4], cy[4], cz[4];
+          int off[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int c = (k + lane) & 3;
+            cm[k] = A(pick4(c, mom0[0], mom0[1], mom0[2], mp));
+            cx[k] = A(pick4(c, aff[0], aff[3], aff[6], R(0)));
+            cy[k] = A(pick4(c, aff[1], aff[4], aff[7], R(0)));
+            cz[k] = A(pick4(c, aff[2], aff[5], aff[8], R(0)));
+            off[k] = c * kN;
+          }
 #pragma unroll
           for (int ii = 0; ii < 3; ++ii) {
             const A dpi = (A(io[ii]) - A(fx[0])) * dx;
-            A mi[3];
+            A mi[4];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) mi[a] = A(mom0[a]) + A(aff[a * 3 + 0]) * dpi;
+            for (int k = 0; k < 4; ++k) mi[k] = cm[k] + cx[k] * dpi;
 #pragma unroll
             for (int jj = 0; jj < 3; ++jj) {
               const A wij = wir[ii] * wjr[jj];
-              A mij[3];
+              A mij[4];
 #pragma unroll
-              for (int a = 0; a < 3; ++a) mij[a] = mi[a] + A(aff[a * 3 + 1]) * dpj[jj];
+              for (int k = 0; k < 4; ++k) mij[k] = mi[k] + cy[k] * dpj[jj];
               A* tr = tile + row[jj] + plane[ii];
 #pragma unroll
               for (int kk = 0; kk < 3; ++kk) {
                 const A wt = wij * wkr[kk];
                 A* tn = tr + ko[kk];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) atomicAdd(tn + a * kN, wt * (mij[a] + ak[a] * dpk[kk]));
-                atomicAdd(tn + 3 * kN, wt * mpa);
+                for (int k = 0; k < 4; ++k) atomicAdd(tn + off[k], wt * (mij[k] + cz[k] * dpk[kk]));
               }
             }
           }
