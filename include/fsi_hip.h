@@ -162,6 +162,13 @@ typedef struct fsi_mpm_buffers {
   const int32_t* chunks;   /* optional P2G work list from fsi_mpm_sort: (first slot,
                               count <= 256) pairs, one workgroup each; every chunk
                               lies in a 4x4x8 cell box */
+  int32_t* zero_boxes;     /* optional scratch for fsi_mpm_substeps with grid_alt:
+                              16 + 16 * zero_boxes_cap int32. Each P2G pass records
+                              the node box of every workgroup, and the G2P pass
+                              that clears that grid zeroes only those boxes instead
+                              of the whole grid. NULL: whole-grid clears */
+  int32_t zero_boxes_cap;  /* workgroups it holds per pass (>= P2G workgroups:
+                              num_chunks, or ceil(num_particles / 256) unsorted) */
 } fsi_mpm_buffers;
 
 /* Coupling inputs/outputs read and written inside the particle passes

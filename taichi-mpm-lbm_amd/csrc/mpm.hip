@@ -408,6 +408,13 @@ __global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf
 // k_p2g): 2 resident workgroups per CU x 256 CUs.
 constexpr int kWideGridLimit = 512;
 
+// Batched substeps record the node box each P2G workgroup scattered into
+// (8 ints per workgroup: x0, y0, z0, tx, ty, tz, 0, 0), so the G2P pass that
+// clears that grid for its next use zeroes only those boxes. A workgroup
+// whose box exceeds this many nodes flags the pass instead, and the whole
+// grid is zeroed.
+constexpr int kZeroBoxMaxNodes = 8192;
+
 // Per-workgroup P2G accumulation tile in LDS: 8 values per node (grid
 // momentum xyz + mass, coupling force density xyz + phi), value-major with a
 // fixed stride of kNodes words, so the value index is an immediate offset of
@@ -513,7 +520,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
                                                 int32_t* __restrict__ cmask,
                                                 int32_t* __restrict__ counters, int clear_force,
                                                 int32_t* __restrict__ zero_ctr,
-                                                const int32_t* __restrict__ chunks) {
+                                                const int32_t* __restrict__ chunks,
+                                                int32_t* __restrict__ boxes_out,
+                                                int32_t* __restrict__ big_flag, int big_value) {
   using A = Tile::acc;
   constexpr int kN = Tile::kNodes;
   __shared__ A tile[kN * Tile::kValues];
@@ -593,6 +602,18 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
   // particles fit, otherwise the global grid/coupling buffers directly
   const bool use_lds = any && tnodes <= kN;
   const int x0 = mn[0], y0 = mn[1], z0 = mn[2];
+  if (boxes_out && threadIdx.x == 0) {
+    // every node this workgroup writes (tile flush, untiled and deferred
+    // scatters) lies in the box of its particles' stencils
+    int32_t* bo = boxes_out + 8 * blockIdx.x;
+    bo[0] = x0;
+    bo[1] = y0;
+    bo[2] = z0;
+    bo[3] = (int)tx;
+    bo[4] = (int)ty;
+    bo[5] = (int)tz;
+    if (tnodes > kZeroBoxMaxNodes) *big_flag = big_value;
+  }
   const int tyi = use_lds ? (int)ty : 0, tzi = use_lds ? (int)tz : 0;
   // the tile spans base..base+2 of every particle of the workgroup, so the
   // scatter needs no per-node bounds test; out-of-lattice entries, and the
@@ -1034,13 +1055,34 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
                                                 int32_t* __restrict__ counters,
                                                 uint4* __restrict__ zero_grid, int64_t zero_n16,
                                                 int32_t* __restrict__ zero_ctr,
-                                                const int32_t* __restrict__ chunks, int num_chunks) {
+                                                const int32_t* __restrict__ chunks, int num_chunks,
+                                                const int32_t* __restrict__ boxes_in,
+                                                const int32_t* __restrict__ big_flag, int big_expect,
+                                                int num_boxes) {
   __shared__ R vtile[kG2PNodes][3];
   __shared__ int red[6][kBlock / 64];
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   // batched substeps: zero the other grid (read by nobody in this pass) for
   // the next P2G pass, and that pass's counters
-  for (int64_t t = gt; t < zero_n16; t += (int64_t)gridDim.x * blockDim.x) zero_grid[t] = make_uint4(0, 0, 0, 0);
+  // (only the boxes the previous P2G pass recorded for it, unless one of
+  // them was too large)
+  if (zero_grid && boxes_in && *big_flag != big_expect) {
+    R* zg = reinterpret_cast<R*>(zero_grid);
+    for (int c = blockIdx.x; c < num_boxes; c += gridDim.x) {
+      const int32_t* bx = boxes_in + 8 * c;
+      const int x0 = bx[0], y0 = bx[1], z0 = bx[2], ty = bx[4], tz = bx[5];
+      const int n = bx[3] * ty * tz;
+      for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int ni = x0 + t / (ty * tz), nj = y0 + (t / tz) % ty, nk = z0 + t % tz;
+        if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
+        R* gp = zg + 4 * (((int64_t)ni * m.ny + nj) * m.nz + nk);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) gp[a] = R(0);
+      }
+    }
+  } else {
+    for (int64_t t = gt; t < zero_n16; t += (int64_t)gridDim.x * blockDim.x) zero_grid[t] = make_uint4(0, 0, 0, 0);
+  }
   if (zero_ctr && gt < 3) zero_ctr[gt] = 0;
   // particle of this lane: the work list of the particle sort (workgroups
   // beyond it only zero), else slot order
@@ -1244,7 +1286,8 @@ int check_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
 template <typename R>
 int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
           const fsi_coupling_buffers* cb, int clear_force, int do_p2g, bool zero_counters,
-          R* grid_target, int32_t* zero_ctr, hipStream_t st) {
+          R* grid_target, int32_t* zero_ctr, hipStream_t st, int32_t* boxes_out = nullptr,
+          int32_t* big_flag = nullptr, int big_value = 0) {
   MpmK<R> m = make_mpm<R>(p);
   CplK<R> cp = make_cpl<R>(c);
   const bool couple = c && c->enabled;
@@ -1261,7 +1304,7 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
       b->active, (R*)b->pforce, grid_target, cb ? cb->solid : nullptr,                          \
       cb ? (const R*)cb->lbm_vel : nullptr, cb ? (R*)cb->cforce : nullptr,                      \
       cb ? (R*)cb->valid_weight : nullptr, cb ? cb->coupling_mask : nullptr,                    \
-      cb ? cb->counters : nullptr, clear_force, zero_ctr, chunks
+      cb ? cb->counters : nullptr, clear_force, zero_ctr, chunks, boxes_out, big_flag, big_value
   // a grid that fits on the chip at once (2 workgroups per CU with the wide
   // gathers' registers) takes the one-batch gathers: its time is one
   // workgroup's latency
@@ -1290,7 +1333,8 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
 template <typename R>
 int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
                const fsi_coupling_buffers* cb, bool zero_counters, const R* grid_src,
-               void* zero_grid, int32_t* zero_ctr, hipStream_t st) {
+               void* zero_grid, int32_t* zero_ctr, hipStream_t st, const int32_t* boxes_in = nullptr,
+               const int32_t* big_flag = nullptr, int big_expect = 0, int num_boxes = 0) {
   MpmK<R> m = make_mpm<R>(p);
   CplK<R> cp = make_cpl<R>(c);
   const bool contact = c && c->contact_enabled;
@@ -1314,7 +1358,8 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
 #define FSI_G2P_ARGS(CONTACT_ON)                                                                   \
   m, cp, (R*)b->x, (R*)b->v, (R*)b->C, b->active, grid_src, CONTACT_ON ? cb->solid : nullptr,      \
       CONTACT_ON ? (const R*)cb->cforce : nullptr, CONTACT_ON ? cb->contact_mask : nullptr,        \
-      CONTACT_ON ? cb->counters : nullptr, (uint4*)zero_grid, n16, zero_ctr, chunks, num_chunks
+      CONTACT_ON ? cb->counters : nullptr, (uint4*)zero_grid, n16, zero_ctr, chunks, num_chunks,    \
+      boxes_in, big_flag, big_expect, num_boxes
   const dim3 g2p_grid((unsigned)blocks), g2p_block(kBlock);
   if (contact && wide)
     hipLaunchKernelGGL((k_g2p<R, true, true>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
@@ -1357,6 +1402,15 @@ int substeps_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   const bool two = b->grid_alt != nullptr && b->grid_alt != b->grid;
   if (two && substeps > 0)
     if (int rc = clear_t<R>(p, b, ctr, st)) return rc;
+  // dirty boxes: P2G pass s records its workgroups' boxes under parity s%2;
+  // G2P pass s >= 1 zeroes the other grid from the boxes of pass s-1 (pass 0
+  // zeroes it whole: its contents are unknown)
+  const int64_t num_wg = p->num_particles == 0 ? 0
+                         : (b->chunks && p->num_chunks > 0) ? (int64_t)p->num_chunks
+                                                            : (int64_t)blocks_for(p->num_particles);
+  const bool boxes = two && b->zero_boxes && num_wg <= (int64_t)b->zero_boxes_cap;
+  auto flag_of = [&](int q) { return b->zero_boxes + q; };
+  auto boxes_of = [&](int q) { return b->zero_boxes + 16 + (int64_t)q * 8 * b->zero_boxes_cap; };
   for (int s = 0; s < substeps; ++s) {
     local.build_fraction = (s == 0 && build_first) ? 1 : 0;
     const fsi_coupling_params* lc = c ? &local : nullptr;
@@ -1364,11 +1418,16 @@ int substeps_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
     if (!two)
       if (int rc = clear_t<R>(p, b, ctr, st)) return rc;
     int32_t* zc_p2g = (two && ctr && s > 0) ? ctr + 3 : nullptr;
-    if (int rc = p2g_t<R>(p, b, lc, cb, clear_force, 1, false, g, zc_p2g, st)) return rc;
+    if (int rc = p2g_t<R>(p, b, lc, cb, clear_force, 1, false, g, zc_p2g, st, boxes ? boxes_of(s & 1) : nullptr,
+                          boxes ? flag_of(s & 1) : nullptr, s + 1))
+      return rc;
     const bool more = s + 1 < substeps;
     void* zg = (two && more) ? (void*)grids[(s + 1) & 1] : nullptr;
     int32_t* zc_g2p = (two && more && ctr) ? ctr : nullptr;
-    if (int rc = grid_g2p_t<R>(p, b, lc, cb, false, g, zg, zc_g2p, st)) return rc;
+    const bool use_boxes = boxes && s >= 1;
+    if (int rc = grid_g2p_t<R>(p, b, lc, cb, false, g, zg, zc_g2p, st, use_boxes ? boxes_of((s + 1) & 1) : nullptr,
+                               use_boxes ? flag_of((s + 1) & 1) : nullptr, s, (int)num_wg))
+      return rc;
   }
   return 0;
 }

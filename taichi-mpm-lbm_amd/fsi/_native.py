@@ -65,7 +65,8 @@ class MpmParams(ctypes.Structure):
 
 class MpmBuffers(ctypes.Structure):
     _fields_ = [("x", _vp), ("v", _vp), ("C", _vp), ("F", _vp), ("mass", _vp), ("volume", _vp),
-                ("active", _vp), ("pforce", _vp), ("grid", _vp), ("grid_alt", _vp), ("chunks", _vp)]
+                ("active", _vp), ("pforce", _vp), ("grid", _vp), ("grid_alt", _vp), ("chunks", _vp),
+                ("zero_boxes", _vp), ("zero_boxes_cap", ctypes.c_int32)]
 
 
 class CouplingParams(ctypes.Structure):

@@ -66,6 +66,11 @@ class MPMSolver3D:
         # second grid: batched substeps alternate the two, each G2P pass
         # zeroing the one the next P2G pass fills
         self.grid_alt = torch.zeros_like(self.grid)
+        # node boxes each batched P2G pass writes (8 int32 per workgroup, two
+        # passes), so the G2P pass clearing that grid zeroes only those boxes;
+        # one entry per possible workgroup: sort chunks or unsorted blocks
+        self._zero_boxes_cap = self._max_workgroups()
+        self._zero_boxes = torch.zeros(16 + 16 * self._zero_boxes_cap, dtype=torch.int32, device=dev)
         self._sum_ws, self._sum_out = nat.reduction_buffers(dev)
         # particle order: slot t holds input particle ids[t]; sort_particles()
         # reorders the slots by cell key (buffers for it are made on first use)
@@ -74,12 +79,15 @@ class MPMSolver3D:
         self._permuted = False
         self._num_chunks = 0  # P2G work list length after a sort (0: slot order)
         self.num_particles = 0
-        self._buffers = nat.MpmBuffers(
-            nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
-            nat.ptr(self.mass), nat.ptr(self.volume), nat.ptr(self.active),
-            nat.ptr(self.particle_force), nat.ptr(self.grid), nat.ptr(self.grid_alt),
-        )
+        self._buffers = self._make_buffers()
         self._initialized = False
+
+    def _max_workgroups(self) -> int:
+        """Upper bound on P2G workgroups: sort chunks (at most cap/256 plus
+        one per occupied cell block plus one) or ceil(cap/256) unsorted."""
+        cap = self.max_particles
+        blocks = ((self.nx + 5) // 4) * ((self.ny + 5) // 4) * ((self.nz + 5) // 4) + 1
+        return cap // 256 + min(blocks, cap) + 1
 
     # ----------------------------------------------------------- set-up --
 
@@ -262,8 +270,7 @@ class MPMSolver3D:
                 raise RuntimeError(nat.load_library().fsi_last_error().decode(errors="replace"))
             ws = torch.empty(max(int(nbytes.value), 1), dtype=torch.uint8, device=dev)
             # P2G work list: at most cap/256 + (cell blocks + 1) chunks
-            blocks = ((self.nx + 5) // 4) * ((self.ny + 5) // 4) * ((self.nz + 5) // 4) + 1
-            max_chunks = cap // 256 + min(blocks, cap) + 1
+            max_chunks = self._max_workgroups()
             chunks = torch.zeros(2 * max_chunks, dtype=torch.int32, device=dev)
             count = torch.zeros(1, dtype=torch.int32, device=dev)
             self._sort = {"alt": alt, "ws": ws, "chunks": chunks, "max_chunks": max_chunks, "count": count}
@@ -277,6 +284,7 @@ class MPMSolver3D:
             nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
             nat.ptr(self.mass), nat.ptr(self.volume), nat.ptr(self.active),
             nat.ptr(self.particle_force), nat.ptr(self.grid), nat.ptr(self.grid_alt), nat.ptr(chunks),
+            nat.ptr(self._zero_boxes), self._zero_boxes_cap,
         )
 
     def sort_particles(self, keys_out: torch.Tensor | None = None,
