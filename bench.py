@@ -157,12 +157,12 @@ def coupled_scene_large(storage="fp32"):
     return cfg, channel_walls(shape), pos
 
 
-def time_coupled_large(steps, warmup):
+def time_coupled_large(steps, warmup, storage="fp32"):
     import torch
 
     import fsi
 
-    cfg, solid, pos = coupled_scene_large()
+    cfg, solid, pos = coupled_scene_large(storage)
     sim = fsi.FSISimulation(cfg)
     sim.initialize_lbm(solid)
     sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
@@ -324,6 +324,10 @@ def main():
         line["coupled_large"] = {
             "workload": "256x128x128 coupled suspension, FP32, 16 elastic spheres, 112 substeps",
             **time_coupled_large(args.large_steps, 1)}
+        # mixed precision: FP16 shifted populations, FP32 MPM and coupling
+        line["coupled_large_fp16"] = {
+            "workload": "256x128x128 coupled suspension, FP16 lattice storage, FP32 MPM, 112 substeps",
+            **time_coupled_large(args.large_steps, 1, storage="fp16")}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(mask)
     print(json.dumps(line), flush=True)
