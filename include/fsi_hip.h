@@ -198,6 +198,9 @@ typedef struct fsi_coupling_buffers {
   int32_t* contact_mask;   /* cap */
   int32_t* counters;       /* 8: unsupported, partial, clipped, contact, damped, ib active,
                               ib clipped, spare */
+  const void* closure_force; /* optional 3*cap real (SoA): learned correction added to
+                              the penalty force of every coupled particle before the
+                              force-norm limit (NULL: the reference's force) */
 } fsi_coupling_buffers;
 
 /* Zero the grid (fsi/mpm3d.py:205-209, clear_grid). */
@@ -255,6 +258,17 @@ int fsi_mpm_sort(const fsi_mpm_params* p, const fsi_mpm_buffers* src, const fsi_
 int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
                 const fsi_coupling_buffers* cb, void* stream);
 
+/* Learned-closure input features of the first num_particles particles, from
+ * the lattice velocity and solid mask and the occupancy phi of the last
+ * coupled step (slot 3 of cb->cforce): X[p * 18 + k] (float), zero rows for
+ * inactive particles. k: 0-2 u - v (u interpolated over the valid nodes as
+ * the coupling does, coupling.py:352-372), 3-11 grad u (row-major, over the
+ * valid nodes), 12 phi, 13-15 -grad(phi)/|grad(phi)| (0 if |grad phi| <
+ * 1e-6), 16 the valid support weight, 17 |u - v|. No counterpart in the
+ * reference. */
+int fsi_closure_features(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_buffers* cb,
+                         float* X, void* stream);
+
 /* Rotation factor R = U V^T of F = U S V^T per particle (fsi/mpm3d.py:245-246)
  * for n matrices; exposed for tests of the in-kernel SVD. */
 int fsi_polar_rotation(int32_t real, int32_t n, const void* F, void* R, void* stream);
@@ -303,6 +317,40 @@ int fsi_reduce_lattice(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const 
  * out: 10 device doubles. Workspace as for fsi_reduce_lattice. */
 int fsi_reduce_particles(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const void* valid_weight,
                          void* workspace, int64_t workspace_doubles, double* out, void* stream);
+
+/* -------------------------------------------------- learned closure ---- */
+
+/* No counterpart in the reference (the north star's learned correction to
+ * the penalty coupling force of fsi/coupling.py:333-406). An MLP
+ * 18 -> 64 -> 64 -> 3 with GELU, BF16 weights and activations, FP32
+ * accumulation on the matrix cores. Parameters: one FP32 vector of
+ * FSI_MLP_PARAMS floats, W1[64][32] (input columns 18..31 zero), b1[64],
+ * W2[64][64], b2[64], W3[16][64] (rows 3..15 zero), b3[16]; layer l computes
+ * z = W h + b. */
+#define FSI_MLP_IN 18
+#define FSI_MLP_HIDDEN 64
+#define FSI_MLP_OUT 3
+#define FSI_MLP_PARAMS 7312
+/* normalisation block of fsi_mlp_infer: mean[18], inv_std[18], lo[18],
+ * hi[18] (envelope, normalised units), out_mean[3], out_std[3] */
+#define FSI_MLP_NORM 78
+
+/* One training batch: forward, loss = mean over rows and the 3 outputs of
+ * (y - T)^2, backward. Rows idx[0..n) of X (n x 18 normalised features,
+ * row-major; idx NULL: rows 0..n) against T (rows x 3). Adds dLoss/dparams
+ * into grads (zero it first) and the squared-error sum into *loss_sum. */
+int fsi_mlp_train_step(int32_t n, const float* X, const float* T, const int32_t* idx, const float* params,
+                       float* grads, double* loss_sum, void* stream);
+/* Adam update of params with moments m, v (FSI_MLP_PARAMS each); step >= 1 */
+int fsi_mlp_adam(float* params, const float* grads, float* m, float* v, int32_t step, float lr, float beta1,
+                 float beta2, float eps, void* stream);
+/* Network outputs for n rows of normalised features: Y (n x 3) */
+int fsi_mlp_forward(int32_t n, const float* X, const float* params, float* Y, void* stream);
+/* Closure forces for n rows of raw features: out[a * stride + r] =
+ * out_mean[a] + out_std[a] * y[a], or 0 when a normalised feature of row r
+ * lies outside [lo, hi] (counted into *fallbacks, a device int). */
+int fsi_mlp_infer(int32_t n, const float* X, const float* params, const float* norm, float* out, int64_t stride,
+                  int32_t* fallbacks, void* stream);
 
 /* ----------------------------------------------------------- common ---- */
 

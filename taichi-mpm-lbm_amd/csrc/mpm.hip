@@ -522,7 +522,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
                                                 int32_t* __restrict__ zero_ctr,
                                                 const int32_t* __restrict__ chunks,
                                                 int32_t* __restrict__ boxes_out,
-                                                int32_t* __restrict__ big_flag, int big_value) {
+                                                int32_t* __restrict__ big_flag, int big_value,
+                                                const R* __restrict__ closure) {
   using A = Tile::acc;
   constexpr int kN = Tile::kNodes;
   __shared__ A tile[kN * Tile::kValues];
@@ -758,6 +759,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
         const R gm = cp.gamma * mp;
 #pragma unroll
         for (int a = 0; a < 3; ++a) pf[a] = gm * dv[a];
+        // learned closure: its correction joins the penalty force before the
+        // force-norm limit (and so also the fluid's reaction)
+        if (closure) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) pf[a] += closure[a * cap + p];
+        }
         if (cp.use_force) {
           const R nrm = sqrt(pf[0] * pf[0] + pf[1] * pf[1] + pf[2] * pf[2]);
           if (nrm > cp.force_limit && nrm > R(0)) {
@@ -1231,6 +1238,84 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
   }
 }
 
+// Learned-closure input features of one particle (no counterpart in the
+// reference): the penalty coupling's support and interpolated lattice
+// velocity (fsi/coupling.py:352-372, valid = in lattice and fluid), the
+// velocity gradient over the same nodes, the interpolated occupancy phi of
+// the last coupled step and its normalised negative gradient (an outward
+// surface normal of the solid), all in lattice units.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_closure_features(MpmK<R> m, const R* __restrict__ x,
+                                                             const R* __restrict__ v,
+                                                             const int32_t* __restrict__ active,
+                                                             const int32_t* __restrict__ solid,
+                                                             const R* __restrict__ lbm_vel,
+                                                             const R* __restrict__ cforce,
+                                                             float* __restrict__ X) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m.num) return;
+  float* out = X + (int64_t)p * FSI_MLP_IN;
+  if (active[p] == 0) {
+    for (int k = 0; k < FSI_MLP_IN; ++k) out[k] = 0.0f;
+    return;
+  }
+  const int cap = m.cap;
+  const int64_t N = (int64_t)m.nx * m.ny * m.nz;
+  R xp[3], vp[3], fx[3], w[3][3], dw[3][3];
+  int base[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    xp[a] = x[a * cap + p];
+    vp[a] = v[a * cap + p];
+  }
+  locate(xp, m.inv_dx, base, fx);
+  bspline(fx, w);
+  // d w / d x of the quadratic B-spline (derivative of fsi/mpm3d.py:227-232)
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    dw[0][a] = (fx[a] - R(1.5)) * m.inv_dx;
+    dw[1][a] = R(-2) * (fx[a] - R(1)) * m.inv_dx;
+    dw[2][a] = (fx[a] - R(0.5)) * m.inv_dx;
+  }
+  R W = R(0), u[3] = {R(0), R(0), R(0)}, G[9] = {}, phi = R(0), gphi[3] = {R(0), R(0), R(0)};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        bool in;
+        const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
+        if (!in) continue;
+        const R wt = w[i][0] * w[j][1] * w[k][2];
+        const R gw[3] = {dw[i][0] * w[j][1] * w[k][2], w[i][0] * dw[j][1] * w[k][2], w[i][0] * w[j][1] * dw[k][2]};
+        const R ph = cforce[4 * node + 3];
+        phi += wt * ph;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) gphi[b] += gw[b] * ph;
+        if (solid[node] != 0) continue;
+        const R un[3] = {lbm_vel[node], lbm_vel[N + node], lbm_vel[2 * N + node]};
+        W += wt;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          u[a] += wt * un[a];
+#pragma unroll
+          for (int b = 0; b < 3; ++b) G[a * 3 + b] += gw[b] * un[a];
+        }
+      }
+  R rel[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) rel[a] = (W > R(0) ? u[a] / W : R(0)) - vp[a];
+  const R gn = sqrt(gphi[0] * gphi[0] + gphi[1] * gphi[1] + gphi[2] * gphi[2]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[a] = (float)rel[a];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) out[3 + a] = (float)G[a];
+  out[12] = (float)phi;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[13 + a] = gn > R(1e-6) ? (float)(-gphi[a] / gn) : 0.0f;
+  out[16] = (float)W;
+  out[17] = (float)sqrt(rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2]);
+}
+
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_contact(MpmK<R> m, CplK<R> cp, const R* __restrict__ x,
                                                     R* __restrict__ v,
@@ -1304,7 +1389,8 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
       b->active, (R*)b->pforce, grid_target, cb ? cb->solid : nullptr,                          \
       cb ? (const R*)cb->lbm_vel : nullptr, cb ? (R*)cb->cforce : nullptr,                      \
       cb ? (R*)cb->valid_weight : nullptr, cb ? cb->coupling_mask : nullptr,                    \
-      cb ? cb->counters : nullptr, clear_force, zero_ctr, chunks, boxes_out, big_flag, big_value
+      cb ? cb->counters : nullptr, clear_force, zero_ctr, chunks, boxes_out, big_flag, big_value,    \
+      cb ? (const R*)cb->closure_force : nullptr
   // a grid that fits on the chip at once (2 workgroups per CU with the wide
   // gathers' registers) takes the one-batch gathers: its time is one
   // workgroup's latency
@@ -1525,6 +1611,25 @@ int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
   int32_t* ctr = counters ? cb->counters : nullptr;
   if (f64) return substeps_t<double>(p, b, c, cb, ctr, substeps, build_first, clear_force, st);
   return substeps_t<float>(p, b, c, cb, ctr, substeps, build_first, clear_force, st);
+}
+
+int fsi_closure_features(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_buffers* cb,
+                         float* X, void* stream) {
+  if (int rc = check_mpm(p, b)) return rc;
+  FSI_REQUIRE(cb && cb->solid && cb->lbm_vel && cb->cforce && X, "closure_features: null buffer");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = p->num_particles;
+  if (n == 0) return 0;
+  if (p->real == FSI_REAL_F64)
+    hipLaunchKernelGGL(k_closure_features<double>, dim3(blocks_for(n)), dim3(kBlock), 0, st, make_mpm<double>(p),
+                       (const double*)b->x, (const double*)b->v, b->active, cb->solid, (const double*)cb->lbm_vel,
+                       (const double*)cb->cforce, X);
+  else
+    hipLaunchKernelGGL(k_closure_features<float>, dim3(blocks_for(n)), dim3(kBlock), 0, st, make_mpm<float>(p),
+                       (const float*)b->x, (const float*)b->v, b->active, cb->solid, (const float*)cb->lbm_vel,
+                       (const float*)cb->cforce, X);
+  FSI_TRY_LAUNCH("closure_features");
+  return 0;
 }
 
 int fsi_polar_rotation(int32_t real, int32_t n, const void* F, void* R, void* stream) {

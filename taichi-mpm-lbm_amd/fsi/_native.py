@@ -83,7 +83,7 @@ class CouplingParams(ctypes.Structure):
 
 class CouplingBuffers(ctypes.Structure):
     _fields_ = [("solid", _vp), ("lbm_vel", _vp), ("cforce", _vp), ("valid_weight", _vp),
-                ("coupling_mask", _vp), ("contact_mask", _vp), ("counters", _vp)]
+                ("coupling_mask", _vp), ("contact_mask", _vp), ("counters", _vp), ("closure_force", _vp)]
 
 
 class ForceAssembly(ctypes.Structure):
@@ -117,7 +117,16 @@ _SIGNATURES = {
     "fsi_reduce_particles": [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
     "fsi_abi_version": [],
     "fsi_fill_zero": [ctypes.c_int64, _vp, _vp],
+    "fsi_closure_features": [_vp, _vp, _vp, _vp, _vp],
+    "fsi_mlp_train_step": [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "fsi_mlp_adam": [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                     ctypes.c_float, _vp],
+    "fsi_mlp_forward": [ctypes.c_int32, _vp, _vp, _vp, _vp],
+    "fsi_mlp_infer": [ctypes.c_int32, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
 }
+
+# learned-closure MLP shapes (FSI_MLP_* of the header)
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS, MLP_NORM = 18, 64, 3, 7312, 78
 
 
 def library_path() -> Path:

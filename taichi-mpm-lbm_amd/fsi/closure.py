@@ -1,0 +1,381 @@
+"""Learned closure for the penalty coupling force (no counterpart in the
+reference: it corrects the force of fsi/coupling.py:333-406).
+
+A coarse coupled run under-resolves the flow around the solid. A small MLP
+(18 -> 64 -> 64 -> 3, GELU, BF16 weights and activations, FP32
+accumulation) maps per-particle features of the coarse state to a correction
+of the per-particle penalty force, learned from pairs of runs of the same
+scene: a coarse run and a 2x-refined one. The refined run is the same flow
+under acoustic scaling (dx and dt halved): lattice velocities are equal,
+the viscosity and the penalty rate in lattice units double and halve,
+elastic moduli and densities are unchanged, and a force in coarse lattice
+units is a quarter of the same force in fine lattice units. Each coarse
+particle is the parent of the 8 fine particles that fill its volume, so the
+target for a coarse particle is (sum of its children's fine forces) / 4 minus
+its own coarse force at the same physical time.
+
+Everything on the device runs in the library's HIP kernels:
+``fsi_closure_features`` (features), ``fsi_mlp_train_step`` (fused forward,
+loss and backward on the matrix cores), ``fsi_mlp_adam``, ``fsi_mlp_infer``
+(correction, or 0 outside the training envelope). The coupler adds the
+correction to the penalty force of every coupled particle
+(``LBMMpmCoupler.set_closure``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+FEATURES = ("u_rel_x", "u_rel_y", "u_rel_z", "du_x/dx", "du_x/dy", "du_x/dz", "du_y/dx", "du_y/dy", "du_y/dz",
+            "du_z/dx", "du_z/dy", "du_z/dz", "phi", "normal_x", "normal_y", "normal_z", "support", "|u_rel|")
+IN, HID, OUT, N_PARAMS, N_NORM = nat.MLP_IN, nat.MLP_HIDDEN, nat.MLP_OUT, nat.MLP_PARAMS, nat.MLP_NORM
+IN_PAD, OUT_PAD = 32, 16
+_O_W1 = 0
+_O_B1 = _O_W1 + HID * IN_PAD
+_O_W2 = _O_B1 + HID
+_O_B2 = _O_W2 + HID * HID
+_O_W3 = _O_B2 + HID
+_O_B3 = _O_W3 + OUT_PAD * HID
+
+
+def init_parameters(seed: int = 0) -> np.ndarray:
+    """He-uniform weights (output layer scaled by 0.1), zero biases, zero
+    padding; the flat FP32 layout of the kernels."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(N_PARAMS, np.float32)
+    w1 = np.zeros((HID, IN_PAD), np.float32)
+    w1[:, :IN] = rng.uniform(-1.0, 1.0, (HID, IN)) * math.sqrt(6.0 / IN)
+    w3 = np.zeros((OUT_PAD, HID), np.float32)
+    w3[:OUT] = rng.uniform(-1.0, 1.0, (OUT, HID)) * math.sqrt(6.0 / HID) * 0.1
+    p[_O_W1:_O_B1] = w1.reshape(-1)
+    p[_O_W2:_O_B2] = (rng.uniform(-1.0, 1.0, (HID, HID)) * math.sqrt(6.0 / HID)).reshape(-1)
+    p[_O_W3:_O_B3] = w3.reshape(-1)
+    return p
+
+
+class ClosureMLP:
+    """Parameters, gradients and Adam moments of the MLP on the device."""
+
+    def __init__(self, params: np.ndarray | None = None, seed: int = 0, device: torch.device | None = None):
+        self.device = device or nat.device()
+        p = init_parameters(seed) if params is None else np.asarray(params, np.float32)
+        if p.shape != (N_PARAMS,):
+            raise ValueError(f"MLP parameters must have shape ({N_PARAMS},), got {p.shape}.")
+        self.params = torch.from_numpy(p.copy()).to(self.device)
+        self.grads = torch.zeros_like(self.params)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self._loss = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self.step_count = 0
+
+    def forward(self, X: torch.Tensor) -> torch.Tensor:
+        """Outputs (n x 3, normalised units) of normalised features X (n x 18)."""
+        X = _rows(X, IN, self.device)
+        Y = torch.empty((X.shape[0], OUT), dtype=torch.float32, device=self.device)
+        nat.call("fsi_mlp_forward", X.shape[0], nat.ptr(X), nat.ptr(self.params), nat.ptr(Y))
+        return Y
+
+    def gradient(self, X: torch.Tensor, T: torch.Tensor, idx: torch.Tensor | None = None) -> float:
+        """Fills self.grads with d(mean squared error)/dparams over the rows
+        idx of (X, T) (all rows when None); returns the mean squared error."""
+        X = _rows(X, IN, self.device)
+        T = _rows(T, OUT, self.device)
+        n = X.shape[0] if idx is None else int(idx.numel())
+        nat.fill_zero(self.grads)
+        nat.fill_zero(self._loss)
+        nat.call("fsi_mlp_train_step", n, nat.ptr(X), nat.ptr(T), nat.ptr(idx), nat.ptr(self.params),
+                 nat.ptr(self.grads), nat.ptr(self._loss))
+        return float(self._loss.item()) / (3.0 * max(n, 1))
+
+    def adam(self, lr: float = 1.0e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1.0e-8) -> None:
+        self.step_count += 1
+        nat.call("fsi_mlp_adam", nat.ptr(self.params), nat.ptr(self.grads), nat.ptr(self.m), nat.ptr(self.v),
+                 self.step_count, float(lr), float(beta1), float(beta2), float(eps))
+
+    def parameters_numpy(self) -> np.ndarray:
+        return self.params.cpu().numpy().copy()
+
+
+def _rows(t, ncols, device) -> torch.Tensor:
+    t = torch.as_tensor(t, dtype=torch.float32, device=device).contiguous()
+    if t.dim() != 2 or t.shape[1] != ncols:
+        raise ValueError(f"expected an (n, {ncols}) array, got {tuple(t.shape)}.")
+    return t
+
+
+@dataclasses.dataclass
+class Normalization:
+    """Feature/target standardisation and the training envelope."""
+
+    mean: np.ndarray
+    std: np.ndarray
+    lo: np.ndarray
+    hi: np.ndarray
+    out_mean: np.ndarray
+    out_std: np.ndarray
+
+    @classmethod
+    def fit(cls, X: np.ndarray, T: np.ndarray, margin: float = 0.1) -> "Normalization":
+        X = np.asarray(X, np.float64)
+        T = np.asarray(T, np.float64)
+        mean, std = X.mean(0), X.std(0)
+        std = np.where(std > 1e-12, std, 1.0)
+        Z = (X - mean) / std
+        lo, hi = Z.min(0), Z.max(0)
+        pad = margin * (hi - lo)
+        omean, ostd = T.mean(0), T.std(0)
+        ostd = np.where(ostd > 1e-20, ostd, 1.0)
+        return cls(mean, std, lo - pad, hi + pad, omean, ostd)
+
+    def packed(self) -> np.ndarray:
+        """The kernels' 78-float block: mean, 1/std, lo, hi, out mean, out std."""
+        return np.concatenate([self.mean, 1.0 / self.std, self.lo, self.hi, self.out_mean,
+                               self.out_std]).astype(np.float32)
+
+    def inputs(self, X) -> np.ndarray:
+        return ((np.asarray(X, np.float64) - self.mean) / self.std).astype(np.float32)
+
+    def targets(self, T) -> np.ndarray:
+        return ((np.asarray(T, np.float64) - self.out_mean) / self.out_std).astype(np.float32)
+
+
+class LearnedClosure:
+    """A trained MLP and its normalisation, evaluated per coupled step."""
+
+    def __init__(self, mlp: ClosureMLP, norm: Normalization):
+        self.mlp = mlp
+        self.norm = norm
+        self._norm_dev = torch.from_numpy(norm.packed()).to(mlp.device)
+        self.fallbacks = torch.zeros(1, dtype=torch.int32, device=mlp.device)
+        self._X: torch.Tensor | None = None
+
+    # -------------------------------------------------------- coupler --
+
+    def features(self, coupler) -> torch.Tensor:
+        """Feature rows (capacity x 18, slot order) of the coupler's particles."""
+        mpm = coupler.mpm
+        cap = mpm.max_particles
+        if self._X is None or self._X.shape[0] != cap:
+            self._X = torch.zeros((cap, IN), dtype=torch.float32, device=self.mlp.device)
+        mp = mpm.params(mpm.dt_default)
+        nat.call("fsi_closure_features", nat.ref(mp), nat.ref(mpm._buffers), nat.ref(coupler._cbuf),
+                 nat.ptr(self._X))
+        return self._X
+
+    def apply(self, coupler, out: torch.Tensor) -> None:
+        """Writes the correction of every particle into `out` (3 x capacity,
+        the coupler's real type; 0 outside the envelope, counted)."""
+        X = self.features(coupler)
+        n = coupler.mpm.num_particles
+        if out.dtype == torch.float32:
+            dst = out
+        else:
+            dst = torch.zeros(3 * coupler.mpm.max_particles, dtype=torch.float32, device=out.device)
+        nat.call("fsi_mlp_infer", n, nat.ptr(X), nat.ptr(self.mlp.params), nat.ptr(self._norm_dev), nat.ptr(dst),
+                 coupler.mpm.max_particles, nat.ptr(self.fallbacks))
+        if dst is not out:
+            out.copy_(dst.to(out.dtype))
+
+    def predict(self, X_raw) -> np.ndarray:
+        """Corrections (n x 3) for raw feature rows (host arrays), with the
+        envelope fallback."""
+        X = _rows(X_raw, IN, self.mlp.device)
+        n = X.shape[0]
+        out = torch.zeros(3 * max(n, 1), dtype=torch.float32, device=self.mlp.device)
+        nat.call("fsi_mlp_infer", n, nat.ptr(X), nat.ptr(self.mlp.params), nat.ptr(self._norm_dev), nat.ptr(out),
+                 max(n, 1), nat.ptr(self.fallbacks))
+        return out.view(3, -1)[:, :n].cpu().numpy().T.copy()
+
+    # ------------------------------------------------------ save/load --
+
+    def save(self, path) -> Path:
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        n = self.norm
+        np.savez(path, params=self.mlp.parameters_numpy(), mean=n.mean, std=n.std, lo=n.lo, hi=n.hi,
+                 out_mean=n.out_mean, out_std=n.out_std)
+        return path
+
+    @classmethod
+    def load(cls, path) -> "LearnedClosure":
+        with np.load(Path(path), allow_pickle=False) as d:
+            norm = Normalization(d["mean"], d["std"], d["lo"], d["hi"], d["out_mean"], d["out_std"])
+            return cls(ClosureMLP(d["params"]), norm)
+
+
+def train_closure(X: np.ndarray, T: np.ndarray, steps: int = 2000, batch: int = 65536, lr: float = 1.0e-3,
+                  seed: int = 0, log_every: int = 0) -> tuple[LearnedClosure, list[float]]:
+    """Fits the MLP to raw features X (n x 18) and targets T (n x 3) with
+    Adam on random batches; returns the closure and the loss history
+    (normalised mean squared error per step)."""
+    norm = Normalization.fit(X, T)
+    mlp = ClosureMLP(seed=seed)
+    Xd = torch.from_numpy(norm.inputs(X)).to(mlp.device)
+    Td = torch.from_numpy(norm.targets(T)).to(mlp.device)
+    n = Xd.shape[0]
+    gen = torch.Generator(device=mlp.device)
+    gen.manual_seed(seed)
+    b = min(batch, n)
+    history = []
+    for s in range(steps):
+        idx = torch.randint(0, n, (b,), device=mlp.device, dtype=torch.int32, generator=gen)
+        loss = mlp.gradient(Xd, Td, idx)
+        mlp.adam(lr)
+        history.append(loss)
+        if log_every and (s % log_every == 0 or s == steps - 1):
+            print(f"closure training step {s}: loss {loss:.4e}", flush=True)
+    return LearnedClosure(mlp, norm), history
+
+
+# ------------------------------------------------------------ scenes --
+
+
+@dataclasses.dataclass(frozen=True)
+class ClosureScene:
+    """One seeded scene of the closure data set: a single elastic ellipsoid
+    in a channel with inflow (the coarse description; `refined` doubles it)."""
+
+    shape: tuple[int, int, int] = (48, 24, 24)
+    center: tuple[float, float, float] = (16.0, 12.0, 12.0)
+    semi_axes: tuple[float, float, float] = (4.0, 3.0, 3.0)
+    rotation: tuple[float, ...] = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+    youngs_modulus: float = 1.0e3
+    poisson_ratio: float = 0.3
+    inflow: float = 0.03
+    viscosity: float = 0.02
+    gamma: float = 0.5
+    substeps: int = 112
+    density: float = 1.2
+    seed: int = 0
+
+    @classmethod
+    def random(cls, seed: int, shape=(48, 24, 24), axes=(3.0, 5.0), modulus=(2.0e2, 2.0e3),
+               poisson=(0.2, 0.4), inflow=(0.01, 0.05)) -> "ClosureScene":
+        """Semi-axes U[axes], random orientation, E log-uniform in `modulus`,
+        nu U[poisson], inflow U[inflow]; centred a third along the channel."""
+        from .scenes import random_rotation
+
+        rng = np.random.default_rng(seed)
+        semi = tuple(float(a) for a in rng.uniform(axes[0], axes[1], 3))
+        R = random_rotation(rng)
+        nx, ny, nz = shape
+        return cls(shape=tuple(shape), center=(nx / 3.0, ny / 2.0, nz / 2.0), semi_axes=semi,
+                   rotation=tuple(float(v) for v in R.reshape(-1)),
+                   youngs_modulus=float(math.exp(rng.uniform(math.log(modulus[0]), math.log(modulus[1])))),
+                   poisson_ratio=float(rng.uniform(*poisson)), inflow=float(rng.uniform(*inflow)), seed=seed)
+
+    def particles(self) -> tuple[np.ndarray, np.ndarray]:
+        """Coarse particles (spacing 0.5, jittered) inside the ellipsoid and
+        their 8 children each for the refined run (fine units: parent x 2
+        +- 0.25), children of parent i at rows 8 i .. 8 i + 7."""
+        rng = np.random.default_rng(self.seed + 7919)
+        c = np.asarray(self.center)
+        semi = np.asarray(self.semi_axes)
+        R = np.asarray(self.rotation).reshape(3, 3)
+        ext = np.abs(R) @ semi
+        axes = [np.arange(np.floor(c[a] - ext[a]) + 0.25, c[a] + ext[a], 0.5) for a in range(3)]
+        grid = np.stack([m.ravel() for m in np.meshgrid(*axes, indexing="ij")], axis=1)
+        local = (grid - c) @ R
+        parents = grid[((local / semi) ** 2).sum(1) <= 1.0]
+        parents = parents + rng.uniform(-0.125, 0.125, parents.shape)
+        off = np.array([[sx, sy, sz] for sx in (-0.25, 0.25) for sy in (-0.25, 0.25) for sz in (-0.25, 0.25)])
+        children = (2.0 * parents[:, None, :] + off[None, :, :]).reshape(-1, 3)
+        return parents.astype(np.float32), children.astype(np.float32)
+
+    def config(self, refined: bool = False, max_particles: int | None = None):
+        """SimulationConfig of the coarse run, or of the 2x-refined run
+        (acoustic scaling: viscosity x 2, penalty rate / 2, same moduli)."""
+        from .config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
+
+        f = 2 if refined else 1
+        nx, ny, nz = (f * s for s in self.shape)
+        lbm = LBMConfig(nx=nx, ny=ny, nz=nz, viscosity=f * self.viscosity, rho0=1.0,
+                        initial_velocity=(self.inflow, 0.0, 0.0),
+                        x_left=BoundaryConfig("velocity", velocity=(self.inflow, 0.0, 0.0)),
+                        x_right=BoundaryConfig("pressure", rho=1.0))
+        npart = max_particles or len(self.particles()[1 if refined else 0])
+        mpm = MPMConfig(nx=nx, ny=ny, nz=nz, max_particles=npart, dx=1.0, dt=1.0 / self.substeps,
+                        density=self.density, youngs_modulus=self.youngs_modulus, poisson_ratio=self.poisson_ratio,
+                        gravity=(0.0, 0.0, 0.0), boundary_width=2)
+        cpl = CouplingConfig(gamma=self.gamma / f, mpm_substeps_per_lbm_step=self.substeps)
+        return SimulationConfig(num_steps=1, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
+
+    def simulation(self, refined: bool = False):
+        from .scenes import channel_walls
+        from .simulation import FSISimulation
+
+        cfg = self.config(refined)
+        sim = FSISimulation(cfg)
+        sim.initialize_lbm(channel_walls((cfg.lbm.nx, cfg.lbm.ny, cfg.lbm.nz)))
+        parents, children = self.particles()
+        sim.initialize_mpm_from_numpy(children if refined else parents, particle_volume=0.125)
+        return sim
+
+
+class _FeatureProbe:
+    """Feature rows of a coupler's particles in input order (host)."""
+
+    def __init__(self, coupler):
+        self.coupler = coupler
+        self.X = torch.zeros((coupler.mpm.max_particles, IN), dtype=torch.float32, device=coupler.mpm.device)
+
+    def features(self) -> np.ndarray:
+        self.coupler.lbm._ensure_macro()
+        mpm = self.coupler.mpm
+        mp = mpm.params(mpm.dt_default)
+        nat.call("fsi_closure_features", nat.ref(mp), nat.ref(mpm._buffers), nat.ref(self.coupler._cbuf),
+                 nat.ptr(self.X))
+        return mpm._input_order(self.X[: mpm.num_particles].cpu().numpy().copy())
+
+
+def scene_pair_data(scene: ClosureScene, steps: int, every: int = 4, warmup: int = 8) -> tuple[np.ndarray, np.ndarray]:
+    """Runs the coarse and refined simulations of a scene side by side and
+    returns (features, targets) of the coarse particles at coarse steps
+    warmup, warmup + every, ... < steps."""
+    coarse = scene.simulation(False)
+    fine = scene.simulation(True)
+    probe = _FeatureProbe(coarse.coupler)
+    Xs, Ts = [], []
+    for k in range(steps):
+        if k >= warmup and (k - warmup) % every == 0:
+            X = probe.features()
+            fc = coarse.mpm.particle_forces_numpy().astype(np.float64)
+            ff = fine.mpm.particle_forces_numpy().astype(np.float64).reshape(-1, 8, 3).sum(1) / 4.0
+            active = coarse.mpm.active_numpy() != 0
+            Xs.append(X[active])
+            Ts.append((ff - fc)[active])
+        coarse.step()
+        fine.step()
+        fine.step()
+    return np.concatenate(Xs).astype(np.float32), np.concatenate(Ts).astype(np.float32)
+
+
+def drag_and_displacement(scene: ClosureScene, steps: int, closure: LearnedClosure | None = None,
+                          refined: bool = False) -> dict:
+    """Mean x-force on the solid over the second half of the run (coarse
+    lattice units: fine forces / 4) and the centroid displacement at the end
+    (coarse cells), of the coarse run (optionally with the closure) or the
+    refined run (twice the steps)."""
+    sim = scene.simulation(refined)
+    if closure is not None:
+        sim.coupler.set_closure(closure)
+    f = 2 if refined else 1
+    x0 = sim.mpm.positions_numpy().astype(np.float64).mean(0) / f
+    drag = []
+    n = steps * f
+    for k in range(n):
+        sim.step()
+        if k >= n // 2:
+            drag.append(float(sim.mpm.particle_forces_numpy().astype(np.float64)[:, 0].sum()))
+    x1 = sim.mpm.positions_numpy().astype(np.float64).mean(0) / f
+    scale = 4.0 if refined else 1.0
+    return {"drag_x": float(np.mean(drag)) / scale, "displacement": (x1 - x0).tolist(),
+            "displacement_norm": float(np.linalg.norm(x1 - x0))}

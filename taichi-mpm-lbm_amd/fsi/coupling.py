@@ -73,6 +73,30 @@ class LBMMpmCoupler:
             nat.ptr(lbm.solid), nat.ptr(lbm.vel), nat.ptr(self.cforce), nat.ptr(self.valid_weight),
             nat.ptr(self.coupling_mask), nat.ptr(self.contact_mask), nat.ptr(self.counters),
         )
+        # optional learned closure (fsi.closure): a per-particle correction to
+        # the penalty force, evaluated once per coupled step
+        self.closure = None
+        self.closure_force: torch.Tensor | None = None
+
+    def set_closure(self, closure) -> None:
+        """Attach a ``fsi.closure.LearnedClosure`` (None detaches it). Each
+        coupled step then evaluates it on the state at the start of the step
+        and adds its correction to every coupled particle's penalty force
+        (and so to the fluid's reaction); particles outside the training
+        envelope keep the plain force. Not part of the reference."""
+        self.closure = closure
+        if closure is None:
+            self.closure_force = None
+            self._cbuf.closure_force = None
+            return
+        cap = self.mpm.max_particles
+        self.closure_force = torch.zeros(3 * cap, dtype=self.lbm.real_dtype, device=self.lbm.device)
+        self._cbuf.closure_force = self.closure_force.data_ptr()
+
+    def closure_fallback_count(self) -> int:
+        """Particle evaluations that left the closure's training envelope (and
+        kept the plain force) since the closure was attached."""
+        return 0 if self.closure is None else int(self.closure.fallbacks.item())
 
     def _validate_solver_contract(self) -> None:
         # fsi/coupling.py:42-56
@@ -157,6 +181,9 @@ class LBMMpmCoupler:
         if self.sort_interval > 0 and self._steps_since_sort % self.sort_interval == 0:
             self.mpm.sort_particles()
         self._steps_since_sort += 1
+        if self.closure is not None and self.enabled:
+            # features read the occupancy of the last step: before the clear
+            self.closure.apply(self, self.closure_force)
         self.clear_coupling_fields()
         mpm_dt = lbm_dt_value / float(self.substeps)
         dt_ratio = mpm_dt / lbm_dt_value
