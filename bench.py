@@ -180,6 +180,70 @@ def time_coupled_large(steps, warmup, storage="fp32"):
             "particle_substeps_per_s": round(len(pos) * n_sub * steps / dt, 1)}
 
 
+def time_closure(train_scenes=8, train_steps=40, eval_steps=60, speed_steps=2):
+    """Learned closure: trains on coarse/2x-refined pairs of single-ellipsoid
+    scenes (48x24x24 coarse), compares drag and centroid displacement of a
+    held-out scene (plain coarse, coarse + closure, refined), then times the
+    coarse run with FP16 populations and the closure against the refined
+    FP32 run on a 64-sphere channel (192x96x96 coarse, 384x192x192 refined)."""
+    import numpy as np
+    import torch
+
+    from fsi.closure import (ClosureScene, drag_and_displacement, pair_config, pair_simulation,
+                             refine_particles, scene_pair_data, train_closure)
+    from fsi.scenes import sphere_pack
+
+    t0 = time.perf_counter()
+    data = [scene_pair_data(ClosureScene.random(s, shape=(48, 24, 24), axes=(3.0, 5.0)), train_steps, every=2,
+                            warmup=6) for s in range(300, 300 + train_scenes)]
+    X = np.concatenate([d[0] for d in data])
+    T = np.concatenate([d[1] for d in data])
+    t1 = time.perf_counter()
+    closure, hist = train_closure(X, T, steps=1500, batch=16384, lr=2e-3, seed=1)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    scene = ClosureScene.random(400, shape=(48, 24, 24), axes=(3.0, 5.0))
+    acc = {}
+    for name, kw in (("plain", {}), ("closure", {"closure": closure}), ("fine", {"refined": True})):
+        acc[name] = drag_and_displacement(scene, eval_steps, **kw)
+    fine = acc["fine"]
+
+    def err(r, key):
+        return round(abs(r[key] / fine[key] - 1.0), 4)
+
+    accuracy = {name: {"drag_rel_err": err(acc[name], "drag_x"),
+                       "displacement_rel_err": err(acc[name], "displacement_norm")} for name in ("plain", "closure")}
+    # speed at the large-scene scale
+    shape = (192, 96, 96)
+    parents, _ = sphere_pack(shape, 64, 2.5, 7.0, spacing=0.5, seed=5, margin=4.0, gap=2.0)
+    children = refine_particles(parents)
+    coarse = pair_simulation(pair_config(shape, False, len(parents), 1.0e3, 0.3, 0.04, storage="fp16"), parents)
+    coarse.coupler.set_closure(closure)
+    fine_sim = pair_simulation(pair_config(shape, True, len(children), 1.0e3, 0.3, 0.04), children)
+
+    def timed(sim, n):
+        sim.step()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(n):
+            sim.step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / n
+
+    tc = timed(coarse, speed_steps)
+    tf = timed(fine_sim, 2 * speed_steps) * 2.0
+    return {"train": {"scenes": train_scenes, "samples": int(len(X)), "adam_steps": 1500,
+                      "data_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+                      "loss_first": round(hist[0], 4), "loss_last": round(hist[-1], 4)},
+            "accuracy_vs_refined": {"scene": "ellipsoid seed 400, 48x24x24 coarse, 60 coarse steps", **accuracy},
+            "speed": {"workload": "64 spheres r 2.5-7 (coarse cells), 192x96x96 FP16 lattice + closure vs "
+                                  "384x192x192 FP32 refined, 112 substeps",
+                      "coarse_particles": int(len(parents)), "fine_particles": int(len(children)),
+                      "coarse_closure_ms_per_step": round(1e3 * tc, 3),
+                      "fine_ms_per_coarse_step": round(1e3 * tf, 3), "speedup": round(tf / tc, 2),
+                      "closure_fallbacks": coarse.coupler.closure_fallback_count()}}
+
+
 def time_coupled(steps, warmup):
     import torch
 
@@ -228,6 +292,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-coupled", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--large-steps", type=int, default=3,
                     help="timed steps of the 256x128x128 coupled suspension (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -328,6 +393,8 @@ def main():
         line["coupled_large_fp16"] = {
             "workload": "256x128x128 coupled suspension, FP16 lattice storage, FP32 MPM, 112 substeps",
             **time_coupled_large(args.large_steps, 1, storage="fp16")}
+    if not args.no_closure:
+        line["learned_closure"] = time_closure()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(mask)
     print(json.dumps(line), flush=True)

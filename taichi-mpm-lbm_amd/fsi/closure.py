@@ -238,6 +238,49 @@ def train_closure(X: np.ndarray, T: np.ndarray, steps: int = 2000, batch: int = 
 # ------------------------------------------------------------ scenes --
 
 
+def refine_particles(parents: np.ndarray) -> np.ndarray:
+    """The 8 particles of the 2x-refined run that fill each coarse particle's
+    volume (coarse spacing 0.5): 2 x parent +- 0.25 in fine units; children
+    of parent i at rows 8 i .. 8 i + 7."""
+    off = np.array([[sx, sy, sz] for sx in (-0.25, 0.25) for sy in (-0.25, 0.25) for sz in (-0.25, 0.25)])
+    p = np.asarray(parents, np.float64)
+    return (2.0 * p[:, None, :] + off[None, :, :]).reshape(-1, 3).astype(np.float32)
+
+
+def pair_config(shape, refined: bool, n_particles: int, youngs_modulus: float, poisson_ratio: float,
+                inflow: float, viscosity: float = 0.02, gamma: float = 0.5, substeps: int = 112,
+                density: float = 1.2, storage: str = "fp32"):
+    """SimulationConfig of a channel (velocity inflow on x-, pressure outflow
+    on x+, fluid starting at the inflow velocity) for the coarse run or its
+    2x refinement (acoustic scaling: viscosity x 2, penalty rate / 2, same
+    moduli, density and lattice velocities)."""
+    from .config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
+
+    f = 2 if refined else 1
+    nx, ny, nz = (f * int(s) for s in shape)
+    lbm = LBMConfig(nx=nx, ny=ny, nz=nz, viscosity=f * viscosity, rho0=1.0, storage=storage,
+                    initial_velocity=(inflow, 0.0, 0.0),
+                    x_left=BoundaryConfig("velocity", velocity=(inflow, 0.0, 0.0)),
+                    x_right=BoundaryConfig("pressure", rho=1.0))
+    mpm = MPMConfig(nx=nx, ny=ny, nz=nz, max_particles=int(n_particles), dx=1.0, dt=1.0 / substeps,
+                    density=density, youngs_modulus=youngs_modulus, poisson_ratio=poisson_ratio,
+                    gravity=(0.0, 0.0, 0.0), boundary_width=2)
+    cpl = CouplingConfig(gamma=gamma / f, mpm_substeps_per_lbm_step=substeps)
+    return SimulationConfig(num_steps=1, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
+
+
+def pair_simulation(cfg, positions):
+    """A simulation of `cfg` with channel walls and the given particles
+    (volume 0.125 each, mass = density x volume)."""
+    from .scenes import channel_walls
+    from .simulation import FSISimulation
+
+    sim = FSISimulation(cfg)
+    sim.initialize_lbm(channel_walls((cfg.lbm.nx, cfg.lbm.ny, cfg.lbm.nz)))
+    sim.initialize_mpm_from_numpy(positions, particle_volume=0.125)
+    return sim
+
+
 @dataclasses.dataclass(frozen=True)
 class ClosureScene:
     """One seeded scene of the closure data set: a single elastic ellipsoid
@@ -274,8 +317,7 @@ class ClosureScene:
 
     def particles(self) -> tuple[np.ndarray, np.ndarray]:
         """Coarse particles (spacing 0.5, jittered) inside the ellipsoid and
-        their 8 children each for the refined run (fine units: parent x 2
-        +- 0.25), children of parent i at rows 8 i .. 8 i + 7."""
+        their children for the refined run (``refine_particles``)."""
         rng = np.random.default_rng(self.seed + 7919)
         c = np.asarray(self.center)
         semi = np.asarray(self.semi_axes)
@@ -285,39 +327,18 @@ class ClosureScene:
         grid = np.stack([m.ravel() for m in np.meshgrid(*axes, indexing="ij")], axis=1)
         local = (grid - c) @ R
         parents = grid[((local / semi) ** 2).sum(1) <= 1.0]
-        parents = parents + rng.uniform(-0.125, 0.125, parents.shape)
-        off = np.array([[sx, sy, sz] for sx in (-0.25, 0.25) for sy in (-0.25, 0.25) for sz in (-0.25, 0.25)])
-        children = (2.0 * parents[:, None, :] + off[None, :, :]).reshape(-1, 3)
-        return parents.astype(np.float32), children.astype(np.float32)
+        parents = (parents + rng.uniform(-0.125, 0.125, parents.shape)).astype(np.float32)
+        return parents, refine_particles(parents)
 
-    def config(self, refined: bool = False, max_particles: int | None = None):
-        """SimulationConfig of the coarse run, or of the 2x-refined run
-        (acoustic scaling: viscosity x 2, penalty rate / 2, same moduli)."""
-        from .config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
+    def config(self, refined: bool = False, storage: str = "fp32"):
+        """SimulationConfig of the coarse run or of the 2x-refined run."""
+        npart = len(self.particles()[1 if refined else 0])
+        return pair_config(self.shape, refined, npart, self.youngs_modulus, self.poisson_ratio, self.inflow,
+                           self.viscosity, self.gamma, self.substeps, self.density, storage)
 
-        f = 2 if refined else 1
-        nx, ny, nz = (f * s for s in self.shape)
-        lbm = LBMConfig(nx=nx, ny=ny, nz=nz, viscosity=f * self.viscosity, rho0=1.0,
-                        initial_velocity=(self.inflow, 0.0, 0.0),
-                        x_left=BoundaryConfig("velocity", velocity=(self.inflow, 0.0, 0.0)),
-                        x_right=BoundaryConfig("pressure", rho=1.0))
-        npart = max_particles or len(self.particles()[1 if refined else 0])
-        mpm = MPMConfig(nx=nx, ny=ny, nz=nz, max_particles=npart, dx=1.0, dt=1.0 / self.substeps,
-                        density=self.density, youngs_modulus=self.youngs_modulus, poisson_ratio=self.poisson_ratio,
-                        gravity=(0.0, 0.0, 0.0), boundary_width=2)
-        cpl = CouplingConfig(gamma=self.gamma / f, mpm_substeps_per_lbm_step=self.substeps)
-        return SimulationConfig(num_steps=1, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
-
-    def simulation(self, refined: bool = False):
-        from .scenes import channel_walls
-        from .simulation import FSISimulation
-
-        cfg = self.config(refined)
-        sim = FSISimulation(cfg)
-        sim.initialize_lbm(channel_walls((cfg.lbm.nx, cfg.lbm.ny, cfg.lbm.nz)))
+    def simulation(self, refined: bool = False, storage: str = "fp32"):
         parents, children = self.particles()
-        sim.initialize_mpm_from_numpy(children if refined else parents, particle_volume=0.125)
-        return sim
+        return pair_simulation(self.config(refined, storage), children if refined else parents)
 
 
 class _FeatureProbe:
