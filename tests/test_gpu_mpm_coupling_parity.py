@@ -278,3 +278,57 @@ def test_coupled_mixed_precision_error(coupled_oracle):
     assert errs["u"] <= 3e-3 and errs["rho"] <= 5e-5, errs
     assert errs["x"] <= 1e-4 and errs["F"] <= 1e-4, errs
     assert np.array_equal(got["cmask"], ref["cmask"])
+
+
+@pytest.fixture(scope="module")
+def long_run_oracle():
+    import bench
+
+    cfg, solid, pos = bench.coupled_scene_32()
+    out = {}
+    for p in ("f32", "f64"):
+        sim = po.OracleSimulation(cfg, p)
+        sim.initialize_lbm(solid)
+        sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+        for _ in range(100):
+            sim.step()
+        out[p] = sim_fields(sim)
+    return out
+
+
+@pytest.mark.timeout(1200)
+@pytest.mark.parametrize("storage", ["fp32", "fp64"])
+def test_coupled_32_scene_100_steps(storage, long_run_oracle):
+    """The reference's CPU-runnable coupled case (32^3 lattice, 4096
+    particles, 112 substeps) over 100 coupled steps (11,200 MPM substeps).
+
+    FP64 build against the FP64 oracle within 1e-7 relative per field: the
+    only difference is the order of the float atomics, amplified by the run
+    (measured: 1e-14 for rho up to 1.4e-8 for the particle force). FP32
+    against FP64 within 4x the reference-order FP32 oracle's own error
+    (+1e-6): over this many substeps the FP32 rounding of either
+    implementation grows to ~1e-3 in u and ~0.1 in v (the reference's own
+    FP32 arithmetic does), so two FP32 realisations differ by a factor
+    around 1-2 from FP64 in either direction. Coupling masks exact."""
+    import bench
+    import fsi
+
+    cfg, solid, pos = bench.coupled_scene_32(storage)
+    sim = fsi.FSISimulation(cfg)
+    sim.initialize_lbm(solid)
+    sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+    for _ in range(100):
+        sim.step()
+    got = sim_fields(sim)
+    ref64, ref32 = long_run_oracle["f64"], long_run_oracle["f32"]
+    bad = []
+    for key in ("dev", "rho", "u", "x", "v", "F", "pforce", "cforce"):
+        e = rel_inf(got[key], ref64[key])
+        e32 = rel_inf(ref32[key], ref64[key])
+        e_vs32 = rel_inf(got[key], ref32[key])
+        print(f"100 steps {storage} {key}: vs f64 {e:.3e}, f32 oracle vs f64 {e32:.3e}, vs f32 oracle {e_vs32:.3e}")
+        bound = 1e-7 if storage == "fp64" else 4.0 * e32 + 1e-6
+        if not e <= bound:
+            bad.append((key, e, bound))
+    assert not bad, bad
+    assert np.array_equal(got["cmask"], ref64["cmask"])
