@@ -163,12 +163,17 @@ typedef struct fsi_mpm_buffers {
                               count <= 256) pairs, one workgroup each; every chunk
                               lies in a 4x4x8 cell box */
   int32_t* zero_boxes;     /* optional scratch for fsi_mpm_substeps with grid_alt:
-                              16 + 16 * zero_boxes_cap int32. Each P2G pass records
-                              the node box of every workgroup, and the G2P pass
-                              that clears that grid zeroes only those boxes instead
-                              of the whole grid. NULL: whole-grid clears */
+                              16 + 24 * zero_boxes_cap int32 (three passes). Each P2G
+                              pass records the node box of every workgroup, and the
+                              pass that clears that grid zeroes only those boxes
+                              instead of the whole grid. NULL: whole-grid clears */
   int32_t zero_boxes_cap;  /* workgroups it holds per pass (>= P2G workgroups:
                               num_chunks, or ceil(num_particles / 256) unsorted) */
+  void* grid_alt2;         /* optional third 4N real grid: with grid_alt, zero_boxes
+                              and (when counters are kept) the coupling buffers'
+                              counter rows, fsi_mpm_substeps runs G2P of substep s-1
+                              and P2G of substep s in one launch (S + 1 launches for
+                              S substeps) */
 } fsi_mpm_buffers;
 
 /* Coupling inputs/outputs read and written inside the particle passes
@@ -201,6 +206,9 @@ typedef struct fsi_coupling_buffers {
   const void* closure_force; /* optional 3*cap real (SoA): learned correction added to
                               the penalty force of every coupled particle before the
                               force-norm limit (NULL: the reference's force) */
+  int32_t* counter_rows;   /* optional scratch for the fused substeps: 8 int32 per
+                              substep (the last substep's row is copied to counters) */
+  int32_t counter_rows_cap; /* substeps it holds */
 } fsi_coupling_buffers;
 
 /* Zero the grid (fsi/mpm3d.py:205-209, clear_grid). */

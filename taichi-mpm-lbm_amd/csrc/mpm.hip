@@ -507,43 +507,29 @@ __device__ NodeBox workgroup_box(bool act, const int base[3], int (*red)[kBlock 
 // WIDE: gather all 27 nodes in one batch (one memory round trip, more
 // registers: for grids that fit on the chip at once); otherwise one i-plane
 // of 9 nodes per batch (fewer registers, more wavefronts per SIMD).
-template <typename R, bool COUPLE, bool P2G, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
-                                                R* __restrict__ v, const R* __restrict__ C,
-                                                R* __restrict__ Fd, const R* __restrict__ mass,
-                                                const R* __restrict__ vol,
-                                                const int32_t* __restrict__ active,
-                                                R* __restrict__ pforce, R* __restrict__ grid,
-                                                const int32_t* __restrict__ solid,
-                                                const R* __restrict__ lbm_vel, R* __restrict__ cforce,
-                                                R* __restrict__ valid_weight,
-                                                int32_t* __restrict__ cmask,
-                                                int32_t* __restrict__ counters, int clear_force,
-                                                int32_t* __restrict__ zero_ctr,
-                                                const int32_t* __restrict__ chunks,
-                                                int32_t* __restrict__ boxes_out,
-                                                int32_t* __restrict__ big_flag, int big_value,
-                                                const R* __restrict__ closure) {
+// P2G pass of one workgroup (fsi/mpm3d.py:234-266 with the coupling kernel
+// of fsi/coupling.py:332-406 fused in front when COUPLE). Lane particle p
+// (live: a particle of this workgroup). REGS: x, v and C come from the
+// caller's registers (the fused G2P+P2G launch) instead of memory.
+template <typename R, bool COUPLE, bool P2G, bool WIDE, bool REGS>
+__device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, int p, bool live, R* __restrict__ x,
+                                         R* __restrict__ v, const R* __restrict__ C, R* __restrict__ Fd,
+                                         const R* __restrict__ mass, const R* __restrict__ vol,
+                                         const int32_t* __restrict__ active, R* __restrict__ pforce,
+                                         R* __restrict__ grid, const int32_t* __restrict__ solid,
+                                         const R* __restrict__ lbm_vel, R* __restrict__ cforce,
+                                         R* __restrict__ valid_weight, int32_t* __restrict__ cmask,
+                                         int32_t* __restrict__ counters, int clear_force,
+                                         int32_t* __restrict__ boxes_out, int32_t* __restrict__ big_flag,
+                                         int big_value, const R* __restrict__ closure, const R* rx, const R* rv,
+                                         const R* rC) {
   using A = Tile::acc;
   constexpr int kN = Tile::kNodes;
   __shared__ A tile[kN * Tile::kValues];
   // coupling: lattice velocity and a fluid flag per node of the box (zero
   // velocity and flag on solid and out-of-lattice nodes)
   __shared__ __align__(16) R ctile[COUPLE ? kN : 1][4];
-  // batched substeps: the G2P-pass counters of this substep start at zero
-  if (zero_ctr && blockIdx.x == 0 && threadIdx.x < 2) zero_ctr[threadIdx.x] = 0;
   __shared__ int red[6][kBlock / 64];
-  // particle of this lane: from the work list (one chunk of at most 256
-  // particles of one 4x4x4 cell block per workgroup), else slot order
-  int p;
-  bool live;
-  if (chunks) {
-    p = chunks[2 * blockIdx.x] + threadIdx.x;
-    live = (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
-  } else {
-    p = blockIdx.x * blockDim.x + threadIdx.x;
-    live = p < m.num;
-  }
   const bool act = live && active[p] != 0;
   const int cap = m.cap;
   const int64_t N = (int64_t)m.nx * m.ny * m.nz;
@@ -558,8 +544,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
   if (act) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      xp[a] = x[a * cap + p];
-      vp[a] = v[a * cap + p];
+      xp[a] = REGS ? rx[a] : x[a * cap + p];
+      vp[a] = REGS ? rv[a] : v[a * cap + p];
     }
     mp = mass[p];
     vp_vol = vol[p];
@@ -866,7 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
     if (act) {
 #pragma unroll
       for (int r = 0; r < 9; ++r) {
-        Cm[r] = C[r * cap + p];
+        Cm[r] = REGS ? rC[r] : C[r * cap + p];
         Fo[r] = Fd[r * cap + p];
       }
       R Fn[9];
@@ -996,6 +982,42 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
   }
 }
 
+template <typename R, bool COUPLE, bool P2G, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
+                                                R* __restrict__ v, const R* __restrict__ C,
+                                                R* __restrict__ Fd, const R* __restrict__ mass,
+                                                const R* __restrict__ vol,
+                                                const int32_t* __restrict__ active,
+                                                R* __restrict__ pforce, R* __restrict__ grid,
+                                                const int32_t* __restrict__ solid,
+                                                const R* __restrict__ lbm_vel, R* __restrict__ cforce,
+                                                R* __restrict__ valid_weight,
+                                                int32_t* __restrict__ cmask,
+                                                int32_t* __restrict__ counters, int clear_force,
+                                                int32_t* __restrict__ zero_ctr,
+                                                const int32_t* __restrict__ chunks,
+                                                int32_t* __restrict__ boxes_out,
+                                                int32_t* __restrict__ big_flag, int big_value,
+                                                const R* __restrict__ closure) {
+  // batched substeps: the G2P-pass counters of this substep start at zero
+  if (zero_ctr && blockIdx.x == 0 && threadIdx.x < 2) zero_ctr[threadIdx.x] = 0;
+  // particle of this lane: from the work list (one chunk of at most 256
+  // particles of two z-adjacent 4x4x4 cell blocks per workgroup), else slot
+  // order
+  int p;
+  bool live;
+  if (chunks) {
+    p = chunks[2 * blockIdx.x] + threadIdx.x;
+    live = (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
+  } else {
+    p = blockIdx.x * blockDim.x + threadIdx.x;
+    live = p < m.num;
+  }
+  p2g_pass<R, COUPLE, P2G, WIDE, false>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid, solid, lbm_vel,
+                                        cforce, valid_weight, cmask, counters, clear_force, boxes_out, big_flag,
+                                        big_value, closure, nullptr, nullptr, nullptr);
+}
+
 // ------------------------------------------------- grid + G2P pass ----
 
 // fsi/mpm3d.py:268-288 (grid_op) for one node, evaluated where G2P reads it:
@@ -1051,28 +1073,16 @@ __device__ __forceinline__ bool contact_test(const MpmK<R>& m, const CplK<R>& cp
 // update, 3 values per node): a work-list chunk spans at most 360.
 constexpr int kG2PNodes = 512;
 
-template <typename R, bool CONTACT, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
-                                                R* __restrict__ v, R* __restrict__ C,
-                                                const int32_t* __restrict__ active,
-                                                const R* __restrict__ grid,
-                                                const int32_t* __restrict__ solid,
-                                                const R* __restrict__ cforce,
-                                                int32_t* __restrict__ contact_mask,
-                                                int32_t* __restrict__ counters,
-                                                uint4* __restrict__ zero_grid, int64_t zero_n16,
-                                                int32_t* __restrict__ zero_ctr,
-                                                const int32_t* __restrict__ chunks, int num_chunks,
-                                                const int32_t* __restrict__ boxes_in,
-                                                const int32_t* __restrict__ big_flag, int big_expect,
-                                                int num_boxes) {
-  __shared__ R vtile[kG2PNodes][3];
-  __shared__ int red[6][kBlock / 64];
+// Batched substeps: zero the grid the next P2G pass fills (read by nobody
+// in this pass): only the node boxes the P2G pass that last wrote it
+// recorded, unless one of them was too large or there are none; then the
+// whole grid (16-byte words), spread over the launch.
+template <typename R>
+__device__ __forceinline__ void zero_grid_pass(const MpmK<R>& m, uint4* __restrict__ zero_grid, int64_t zero_n16,
+                                               const int32_t* __restrict__ boxes_in,
+                                               const int32_t* __restrict__ big_flag, int big_expect,
+                                               int num_boxes) {
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  // batched substeps: zero the other grid (read by nobody in this pass) for
-  // the next P2G pass, and that pass's counters
-  // (only the boxes the previous P2G pass recorded for it, unless one of
-  // them was too large)
   if (zero_grid && boxes_in && *big_flag != big_expect) {
     R* zg = reinterpret_cast<R*>(zero_grid);
     for (int c = blockIdx.x; c < num_boxes; c += gridDim.x) {
@@ -1090,19 +1100,20 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
   } else {
     for (int64_t t = gt; t < zero_n16; t += (int64_t)gridDim.x * blockDim.x) zero_grid[t] = make_uint4(0, 0, 0, 0);
   }
-  if (zero_ctr && gt < 3) zero_ctr[gt] = 0;
-  // particle of this lane: the work list of the particle sort (workgroups
-  // beyond it only zero), else slot order
-  int p;
-  bool live;
-  if (chunks) {
-    const bool has = (int)blockIdx.x < num_chunks;
-    p = has ? chunks[2 * blockIdx.x] + threadIdx.x : 0;
-    live = has && (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
-  } else {
-    p = gt;
-    live = p < m.num;
-  }
+}
+
+// Grid update + G2P pass of one workgroup (fsi/mpm3d.py:268-328, + the
+// contact pass of fsi/coupling.py:463-496 when CONTACT) for lane particle p.
+// Stores x, v, C and returns them in xo, vo, Co (zeros for lanes without an
+// active particle) for a P2G pass in the same launch.
+template <typename R, bool CONTACT, bool WIDE>
+__device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, int p, bool live, R* __restrict__ x,
+                                         R* __restrict__ v, R* __restrict__ C, const int32_t* __restrict__ active,
+                                         const R* __restrict__ grid, const int32_t* __restrict__ solid,
+                                         const R* __restrict__ cforce, int32_t* __restrict__ contact_mask,
+                                         int32_t* __restrict__ counters, R xo[3], R vo[3], R Co[9]) {
+  __shared__ R vtile[kG2PNodes][3];
+  __shared__ int red[6][kBlock / 64];
   const bool act = live && active[p] != 0;
   const int cap = m.cap;
   bool cand = false, damped = false;
@@ -1230,12 +1241,105 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a * cap + p] = nv[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      xo[a] = xp[a];
+      vo[a] = nv[a];
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) Co[r] = m.apic ? nc[r] : R(0);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xo[a] = vo[a] = R(0);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) Co[r] = R(0);
   }
   if constexpr (CONTACT) {
     if (live) contact_mask[p] = cand ? 1 : 0;
     wave_count(counters + 3, cand);
     wave_count(counters + 4, damped);
   }
+}
+
+template <typename R, bool CONTACT, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
+                                                R* __restrict__ v, R* __restrict__ C,
+                                                const int32_t* __restrict__ active,
+                                                const R* __restrict__ grid,
+                                                const int32_t* __restrict__ solid,
+                                                const R* __restrict__ cforce,
+                                                int32_t* __restrict__ contact_mask,
+                                                int32_t* __restrict__ counters,
+                                                uint4* __restrict__ zero_grid, int64_t zero_n16,
+                                                int32_t* __restrict__ zero_ctr,
+                                                const int32_t* __restrict__ chunks, int num_chunks,
+                                                const int32_t* __restrict__ boxes_in,
+                                                const int32_t* __restrict__ big_flag, int big_expect,
+                                                int num_boxes) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  zero_grid_pass<R>(m, zero_grid, zero_n16, boxes_in, big_flag, big_expect, num_boxes);
+  if (zero_ctr && gt < 3) zero_ctr[gt] = 0;
+  // particle of this lane: the work list of the particle sort (workgroups
+  // beyond it only zero), else slot order
+  int p;
+  bool live;
+  if (chunks) {
+    const bool has = (int)blockIdx.x < num_chunks;
+    p = has ? chunks[2 * blockIdx.x] + threadIdx.x : 0;
+    live = has && (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
+  } else {
+    p = gt;
+    live = p < m.num;
+  }
+  R xo[3], vo[3], Co[9];
+  g2p_pass<R, CONTACT, WIDE>(m, cp, p, live, x, v, C, active, grid, solid, cforce, contact_mask, counters, xo, vo, Co);
+}
+
+// Substeps s-1 and s back to back in one launch: the grid update + G2P of
+// substep s-1 (from grid_src), then, in the same workgroup and with the
+// particle's new x, v and C still in registers, the P2G of substep s (into
+// grid_dst, zeroed before the launch). The work list is the same for both
+// passes; the zeroing part prepares the grid of substep s+1. Workgroups
+// beyond the particle blocks only zero.
+template <typename R, bool COUPLE, bool CONTACT, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_g2p_p2g(
+    MpmK<R> m, CplK<R> cp, R* __restrict__ x, R* __restrict__ v, R* __restrict__ C, R* __restrict__ Fd,
+    const R* __restrict__ mass, const R* __restrict__ vol, const int32_t* __restrict__ active,
+    R* __restrict__ pforce, const R* __restrict__ grid_src, R* __restrict__ grid_dst,
+    const int32_t* __restrict__ solid, const R* __restrict__ lbm_vel, R* __restrict__ cforce,
+    R* __restrict__ valid_weight, int32_t* __restrict__ cmask, int32_t* __restrict__ contact_mask,
+    int32_t* __restrict__ g2p_counters, int32_t* __restrict__ p2g_counters, int clear_force,
+    const int32_t* __restrict__ chunks, int particle_blocks, uint4* __restrict__ zero_grid, int64_t zero_n16,
+    const int32_t* __restrict__ boxes_in, const int32_t* __restrict__ big_in, int big_expect, int num_boxes,
+    int32_t* __restrict__ boxes_out, int32_t* __restrict__ big_out, int big_value, const R* __restrict__ closure) {
+  zero_grid_pass<R>(m, zero_grid, zero_n16, boxes_in, big_in, big_expect, num_boxes);
+  const bool has = (int)blockIdx.x < particle_blocks;
+  int p;
+  bool live;
+  if (chunks) {
+    p = has ? chunks[2 * blockIdx.x] + threadIdx.x : 0;
+    live = has && (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
+  } else {
+    p = blockIdx.x * blockDim.x + threadIdx.x;
+    live = p < m.num;
+  }
+  R xo[3], vo[3], Co[9];
+  g2p_pass<R, CONTACT, WIDE>(m, cp, p, live, x, v, C, active, grid_src, solid, cforce, contact_mask, g2p_counters,
+                             xo, vo, Co);
+  if (has)  // uniform over the workgroup
+    p2g_pass<R, COUPLE, true, WIDE, true>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid_dst, solid,
+                                          lbm_vel, cforce, valid_weight, cmask, p2g_counters, clear_force, boxes_out,
+                                          big_out, big_value, closure, xo, vo, Co);
+}
+
+__global__ void k_fill_i32(int n, int32_t* __restrict__ a, int32_t value) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) a[t] = value;
+}
+
+__global__ void k_copy_i32(int n, const int32_t* __restrict__ src, int32_t* __restrict__ dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
 }
 
 // Learned-closure input features of one particle (no counterpart in the
@@ -1372,8 +1476,9 @@ template <typename R>
 int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
           const fsi_coupling_buffers* cb, int clear_force, int do_p2g, bool zero_counters,
           R* grid_target, int32_t* zero_ctr, hipStream_t st, int32_t* boxes_out = nullptr,
-          int32_t* big_flag = nullptr, int big_value = 0) {
+          int32_t* big_flag = nullptr, int big_value = 0, int32_t* counters_override = nullptr) {
   MpmK<R> m = make_mpm<R>(p);
+  int32_t* ctrs = counters_override ? counters_override : (cb ? cb->counters : nullptr);
   CplK<R> cp = make_cpl<R>(c);
   const bool couple = c && c->enabled;
   if (couple && zero_counters) {
@@ -1389,7 +1494,7 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
       b->active, (R*)b->pforce, grid_target, cb ? cb->solid : nullptr,                          \
       cb ? (const R*)cb->lbm_vel : nullptr, cb ? (R*)cb->cforce : nullptr,                      \
       cb ? (R*)cb->valid_weight : nullptr, cb ? cb->coupling_mask : nullptr,                    \
-      cb ? cb->counters : nullptr, clear_force, zero_ctr, chunks, boxes_out, big_flag, big_value,    \
+      ctrs, clear_force, zero_ctr, chunks, boxes_out, big_flag, big_value,                          \
       cb ? (const R*)cb->closure_force : nullptr
   // a grid that fits on the chip at once (2 workgroups per CU with the wide
   // gathers' registers) takes the one-batch gathers: its time is one
@@ -1420,8 +1525,10 @@ template <typename R>
 int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
                const fsi_coupling_buffers* cb, bool zero_counters, const R* grid_src,
                void* zero_grid, int32_t* zero_ctr, hipStream_t st, const int32_t* boxes_in = nullptr,
-               const int32_t* big_flag = nullptr, int big_expect = 0, int num_boxes = 0) {
+               const int32_t* big_flag = nullptr, int big_expect = 0, int num_boxes = 0,
+               int32_t* counters_override = nullptr) {
   MpmK<R> m = make_mpm<R>(p);
+  int32_t* ctrs = counters_override ? counters_override : (cb ? cb->counters : nullptr);
   CplK<R> cp = make_cpl<R>(c);
   const bool contact = c && c->contact_enabled;
   if (contact && zero_counters) {
@@ -1444,7 +1551,7 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
 #define FSI_G2P_ARGS(CONTACT_ON)                                                                   \
   m, cp, (R*)b->x, (R*)b->v, (R*)b->C, b->active, grid_src, CONTACT_ON ? cb->solid : nullptr,      \
       CONTACT_ON ? (const R*)cb->cforce : nullptr, CONTACT_ON ? cb->contact_mask : nullptr,        \
-      CONTACT_ON ? cb->counters : nullptr, (uint4*)zero_grid, n16, zero_ctr, chunks, num_chunks,    \
+      CONTACT_ON ? ctrs : nullptr, (uint4*)zero_grid, n16, zero_ctr, chunks, num_chunks,            \
       boxes_in, big_flag, big_expect, num_boxes
   const dim3 g2p_grid((unsigned)blocks), g2p_block(kBlock);
   if (contact && wide)
@@ -1473,6 +1580,94 @@ int clear_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, int32_t* counters
   return 0;
 }
 
+// Full clear of one grid.
+template <typename R>
+int clear_grid_t(const fsi_mpm_params* p, R* grid, hipStream_t st) {
+  const int64_t n = 4LL * p->nx * p->ny * p->nz;
+  hipLaunchKernelGGL(k_clear<R>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, grid, nullptr);
+  FSI_TRY_LAUNCH("mpm_clear_grid");
+  return 0;
+}
+
+// Substeps with three grids, S + 1 launches: P2G of substep 0, then for
+// s = 1..S-1 one launch of G2P(s-1) + P2G(s) (k_g2p_p2g), then G2P(S-1).
+// Substep s scatters into grid s%3; the launch of substep s zeroes grid
+// (s+1)%3, which the launch of s-1 finished reading, from the boxes the P2G
+// of substep s-2 recorded (whole grids on the first launches). Counters of
+// substep s go to row s of cb->counter_rows (zeroed once); the last row is
+// copied to cb->counters at the end.
+template <typename R>
+int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+                     const fsi_coupling_buffers* cb, int32_t* ctr, int substeps, int build_first, int clear_force,
+                     hipStream_t st, int64_t num_wg) {
+  fsi_coupling_params local{};
+  if (c) local = *c;
+  R* G[3] = {(R*)b->grid, (R*)b->grid_alt, (R*)b->grid_alt2};
+  int32_t* rows = ctr ? cb->counter_rows : nullptr;
+  auto row = [&](int s) { return rows ? rows + 8 * s : nullptr; };
+  auto flag_of = [&](int q) { return b->zero_boxes + q; };
+  auto boxes_of = [&](int q) { return b->zero_boxes + 16 + (int64_t)q * 8 * b->zero_boxes_cap; };
+  if (int rc = clear_grid_t<R>(p, G[0], st)) return rc;
+  if (int rc = clear_grid_t<R>(p, G[1], st)) return rc;
+  if (rows) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(8 * substeps)), dim3(kBlock), 0, st, 8 * substeps, rows, 0);
+    FSI_TRY_LAUNCH("counter_rows");
+  }
+  local.build_fraction = build_first ? 1 : 0;
+  if (int rc = p2g_t<R>(p, b, c ? &local : nullptr, cb, clear_force, 1, false, G[0], nullptr, st, boxes_of(0),
+                        flag_of(0), 1, row(0)))
+    return rc;
+  local.build_fraction = 0;
+  const fsi_coupling_params* lc = c ? &local : nullptr;
+  MpmK<R> m = make_mpm<R>(p);
+  CplK<R> cp = make_cpl<R>(lc);
+  const bool couple = lc && lc->enabled;
+  const bool contact = lc && lc->contact_enabled;
+  const int32_t* chunks = (b->chunks && p->num_chunks > 0) ? b->chunks : nullptr;
+  const int64_t n16 = (4LL * p->nx * p->ny * p->nz * (int64_t)sizeof(R)) / 16;
+  const int64_t zero_blocks = std::min<int64_t>(blocks_for(n16), 1024);
+  const dim3 grid((unsigned)std::max<int64_t>(num_wg, zero_blocks)), block(kBlock);
+  const bool wide = num_wg <= kWideGridLimit;
+  for (int s = 1; s < substeps; ++s) {
+    const int q = (s + 1) % 3;
+    const bool by_boxes = s >= 2;
+#define FSI_FUSED_ARGS                                                                                          \
+  m, cp, (R*)b->x, (R*)b->v, (R*)b->C, (R*)b->F, (const R*)b->mass, (const R*)b->volume, b->active,            \
+      (R*)b->pforce, (const R*)G[(s - 1) % 3], G[s % 3], cb ? cb->solid : nullptr,                              \
+      cb ? (const R*)cb->lbm_vel : nullptr, cb ? (R*)cb->cforce : nullptr, cb ? (R*)cb->valid_weight : nullptr, \
+      cb ? cb->coupling_mask : nullptr, cb ? cb->contact_mask : nullptr, row(s - 1), row(s), clear_force,      \
+      chunks, (int)num_wg, (uint4*)G[q], n16, by_boxes ? boxes_of(q) : nullptr, by_boxes ? flag_of(q) : nullptr, \
+      s - 1, (int)num_wg, boxes_of(s % 3), flag_of(s % 3), s + 1, cb ? (const R*)cb->closure_force : nullptr
+#define FSI_FUSED_LAUNCH(CPL, CON)                                                                      \
+  do {                                                                                                  \
+    if (wide)                                                                                           \
+      hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, true>), grid, block, 0, st, FSI_FUSED_ARGS);           \
+    else                                                                                                \
+      hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false>), grid, block, 0, st, FSI_FUSED_ARGS);          \
+  } while (0)
+    if (couple && contact)
+      FSI_FUSED_LAUNCH(true, true);
+    else if (couple)
+      FSI_FUSED_LAUNCH(true, false);
+    else if (contact)
+      FSI_FUSED_LAUNCH(false, true);
+    else
+      FSI_FUSED_LAUNCH(false, false);
+#undef FSI_FUSED_LAUNCH
+#undef FSI_FUSED_ARGS
+    FSI_TRY_LAUNCH("mpm_g2p_p2g");
+  }
+  const int last = substeps - 1;
+  if (int rc = grid_g2p_t<R>(p, b, lc, cb, false, G[last % 3], nullptr, nullptr, st, nullptr, nullptr, 0, 0,
+                             row(last)))
+    return rc;
+  if (rows) {
+    hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(64), 0, st, 5, row(last), cb->counters);
+    FSI_TRY_LAUNCH("counter_rows_copy");
+  }
+  return 0;
+}
+
 // fsi/mpm3d.py:189-214 repeated `substeps` times. With one grid: clear
 // (+ counters), P2G (+ coupling), grid update + G2P (+ contact) per substep.
 // With a second grid the clear runs once: substep s scatters into grid s%2,
@@ -1495,6 +1690,12 @@ int substeps_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
                          : (b->chunks && p->num_chunks > 0) ? (int64_t)p->num_chunks
                                                             : (int64_t)blocks_for(p->num_particles);
   const bool boxes = two && b->zero_boxes && num_wg <= (int64_t)b->zero_boxes_cap;
+  // three grids, box scratch for three passes and counter rows: one launch
+  // per substep boundary
+  const bool fused = boxes && b->grid_alt2 && b->grid_alt2 != b->grid && b->grid_alt2 != b->grid_alt &&
+                     substeps >= 2 && num_wg > 0 &&
+                     (!ctr || (cb && cb->counter_rows && cb->counter_rows_cap >= substeps));
+  if (fused) return substeps_fused_t<R>(p, b, c, cb, ctr, substeps, build_first, clear_force, st, num_wg);
   auto flag_of = [&](int q) { return b->zero_boxes + q; };
   auto boxes_of = [&](int q) { return b->zero_boxes + 16 + (int64_t)q * 8 * b->zero_boxes_cap; };
   for (int s = 0; s < substeps; ++s) {

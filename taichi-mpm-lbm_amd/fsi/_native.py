@@ -66,7 +66,7 @@ class MpmParams(ctypes.Structure):
 class MpmBuffers(ctypes.Structure):
     _fields_ = [("x", _vp), ("v", _vp), ("C", _vp), ("F", _vp), ("mass", _vp), ("volume", _vp),
                 ("active", _vp), ("pforce", _vp), ("grid", _vp), ("grid_alt", _vp), ("chunks", _vp),
-                ("zero_boxes", _vp), ("zero_boxes_cap", ctypes.c_int32)]
+                ("zero_boxes", _vp), ("zero_boxes_cap", ctypes.c_int32), ("grid_alt2", _vp)]
 
 
 class CouplingParams(ctypes.Structure):
@@ -83,7 +83,8 @@ class CouplingParams(ctypes.Structure):
 
 class CouplingBuffers(ctypes.Structure):
     _fields_ = [("solid", _vp), ("lbm_vel", _vp), ("cforce", _vp), ("valid_weight", _vp),
-                ("coupling_mask", _vp), ("contact_mask", _vp), ("counters", _vp), ("closure_force", _vp)]
+                ("coupling_mask", _vp), ("contact_mask", _vp), ("counters", _vp), ("closure_force", _vp),
+                ("counter_rows", _vp), ("counter_rows_cap", ctypes.c_int32)]
 
 
 class ForceAssembly(ctypes.Structure):

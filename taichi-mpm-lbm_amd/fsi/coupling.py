@@ -69,9 +69,12 @@ class LBMMpmCoupler:
         # a compact box; results change only by float rounding of the sums
         self.sort_interval = 1
         self._steps_since_sort = 0
+        # per-substep counter rows of the fused substep launches
+        self._counter_rows = torch.zeros(8 * max(self.substeps, 1), dtype=torch.int32, device=dev)
         self._cbuf = nat.CouplingBuffers(
             nat.ptr(lbm.solid), nat.ptr(lbm.vel), nat.ptr(self.cforce), nat.ptr(self.valid_weight),
-            nat.ptr(self.coupling_mask), nat.ptr(self.contact_mask), nat.ptr(self.counters),
+            nat.ptr(self.coupling_mask), nat.ptr(self.contact_mask), nat.ptr(self.counters), None,
+            nat.ptr(self._counter_rows), max(self.substeps, 1),
         )
         # optional learned closure (fsi.closure): a per-particle correction to
         # the penalty force, evaluated once per coupled step

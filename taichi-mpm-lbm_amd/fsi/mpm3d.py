@@ -66,11 +66,15 @@ class MPMSolver3D:
         # second grid: batched substeps alternate the two, each G2P pass
         # zeroing the one the next P2G pass fills
         self.grid_alt = torch.zeros_like(self.grid)
-        # node boxes each batched P2G pass writes (8 int32 per workgroup, two
-        # passes), so the G2P pass clearing that grid zeroes only those boxes;
-        # one entry per possible workgroup: sort chunks or unsorted blocks
+        # third grid: batched substeps then run G2P of one substep and P2G of
+        # the next in one launch, rotating the three grids
+        self.grid_alt2 = torch.zeros_like(self.grid)
+        # node boxes each batched P2G pass writes (8 int32 per workgroup,
+        # three passes), so the pass clearing that grid zeroes only those
+        # boxes; one entry per possible workgroup: sort chunks or unsorted
+        # blocks
         self._zero_boxes_cap = self._max_workgroups()
-        self._zero_boxes = torch.zeros(16 + 16 * self._zero_boxes_cap, dtype=torch.int32, device=dev)
+        self._zero_boxes = torch.zeros(16 + 24 * self._zero_boxes_cap, dtype=torch.int32, device=dev)
         self._sum_ws, self._sum_out = nat.reduction_buffers(dev)
         # particle order: slot t holds input particle ids[t]; sort_particles()
         # reorders the slots by cell key (buffers for it are made on first use)
@@ -284,7 +288,7 @@ class MPMSolver3D:
             nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
             nat.ptr(self.mass), nat.ptr(self.volume), nat.ptr(self.active),
             nat.ptr(self.particle_force), nat.ptr(self.grid), nat.ptr(self.grid_alt), nat.ptr(chunks),
-            nat.ptr(self._zero_boxes), self._zero_boxes_cap,
+            nat.ptr(self._zero_boxes), self._zero_boxes_cap, nat.ptr(self.grid_alt2),
         )
 
     def sort_particles(self, keys_out: torch.Tensor | None = None,
