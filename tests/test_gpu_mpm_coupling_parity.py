@@ -332,3 +332,41 @@ def test_coupled_32_scene_100_steps(storage, long_run_oracle):
             bad.append((key, e, bound))
     assert not bad, bad
     assert np.array_equal(got["cmask"], ref64["cmask"])
+
+
+@pytest.mark.parametrize("path", ["two_grid", "one_grid"])
+def test_coupled_substep_paths_without_third_grid(path, coupled_oracle):
+    """The substep schedules kept for callers without the third grid (two
+    grids, two launches per substep) or without any spare grid (a clear per
+    substep) give the same results as the fused one: FP64 against the FP64
+    oracle, counters exact."""
+    import fsi
+
+    cfg, solid, pos, vel = coupled_scene(storage="fp64", ib=True, contact=True)
+    sim = fsi.FSISimulation(cfg)
+    sim.initialize_lbm(solid)
+    sim.initialize_mpm_from_numpy(pos, vel, particle_mass=1.2 * 0.125, particle_volume=0.125)
+    mpm = sim.mpm
+    make = mpm._make_buffers
+
+    def patched():
+        b = make()
+        b.grid_alt2 = None
+        if path == "one_grid":
+            b.grid_alt = None
+        return b
+
+    mpm._make_buffers = patched
+    mpm._buffers = patched()
+    for _ in range(COUPLED["penalty_ib_contact"]["steps"]):
+        sim.step()
+    got = sim_fields(sim)
+    got["diag"] = sim.coupler.coupling_diagnostics()
+    ref = coupled_oracle["penalty_ib_contact"]["f64"]
+    for key in FIELDS:
+        ok, err, diff = close(got[key], ref[key], 1e-9, 1e-13)
+        print(f"{path} fp64 {key}: rel {err:.3e} abs {diff:.3e}")
+        assert ok, (path, key, err, diff)
+    for k in ("unsupported_particle_count", "partial_support_particle_count", "clipped_particle_count",
+              "ib_active_cell_count", "contact_candidate_count", "contact_damped_particle_count"):
+        assert got["diag"][k] == ref["diag"][k], (k, got["diag"][k], ref["diag"][k])
