@@ -345,10 +345,14 @@ int fsi_reduce_particles(const fsi_mpm_params* p, const fsi_mpm_buffers* b, cons
 
 /* One training batch: forward, loss = mean over rows and the 3 outputs of
  * (y - T)^2, backward. Rows idx[0..n) of X (n x 18 normalised features,
- * row-major; idx NULL: rows 0..n) against T (rows x 3). Adds dLoss/dparams
- * into grads (zero it first) and the squared-error sum into *loss_sum. */
+ * row-major; idx NULL: rows 0..n) against T (rows x 3). Writes
+ * dLoss/dparams to grads (FSI_MLP_PARAMS floats; each wavefront's partial
+ * sums go to `workspace`, then are added in a fixed order, so a batch gives
+ * the same gradient bit for bit) and adds the squared-error sum into
+ * *loss_sum. workspace: at least fsi_mlp_train_workspace(n) floats. */
+int fsi_mlp_train_workspace(int32_t n, int64_t* floats);
 int fsi_mlp_train_step(int32_t n, const float* X, const float* T, const int32_t* idx, const float* params,
-                       float* grads, double* loss_sum, void* stream);
+                       float* grads, double* loss_sum, float* workspace, int64_t workspace_floats, void* stream);
 /* Adam update of params with moments m, v (FSI_MLP_PARAMS each); step >= 1 */
 int fsi_mlp_adam(float* params, const float* grads, float* m, float* v, int32_t step, float lr, float beta1,
                  float beta2, float eps, void* stream);

@@ -50,11 +50,32 @@ __device__ __forceinline__ short bf16_bits(float f) {
 
 __device__ __forceinline__ float bf16_value(short b) { return __uint_as_float((uint32_t)(uint16_t)b << 16); }
 
-// GELU(z) = z Phi(z) and its derivative Phi(z) + z phi(z), Phi the standard
-// normal CDF (exact erf form)
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU(z) = z Phi(z) and its derivative Phi(z) + z phi(z), with Phi the
+// standard normal CDF, Phi(z) = (1 + erf(z / sqrt 2)) / 2, and phi its
+// density. erf by Abramowitz and Stegun, Handbook of Mathematical Functions,
+// 7.1.26 (|error| <= 1.5e-7, far below the BF16 rounding of the
+// activations): erf(x) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
+// exp(-x^2), t = 1 / (1 + p x), x >= 0, odd in x. exp(-x^2) at x = z/sqrt 2
+// is the exp(-z^2/2) of phi, so one exponential serves both.
+__device__ __forceinline__ void normal_cdf_pdf(float z, float& cdf, float& pdf) {
+  const float x = fabsf(z) * 0.70710678118654752f;
+  const float t = 1.0f / (1.0f + 0.3275911f * x);
+  const float e = __expf(-0.5f * z * z);
+  const float poly =
+      t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  const float erf_abs = 1.0f - poly * e;
+  cdf = 0.5f * (1.0f + copysignf(erf_abs, z));
+  pdf = 0.39894228040143268f * e;
+}
+__device__ __forceinline__ float gelu(float z) {
+  float cdf, pdf;
+  normal_cdf_pdf(z, cdf, pdf);
+  return z * cdf;
+}
 __device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+  float cdf, pdf;
+  normal_cdf_pdf(z, cdf, pdf);
+  return cdf + z * pdf;
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -193,10 +214,10 @@ __device__ __forceinline__ void load_inputs(const float* __restrict__ X, int64_t
 
 // Training step: forward, mean squared error over the real outputs, and
 // backward for the batch rows idx[0..n) (or rows 0..n) of X (normalised
-// features, row-major n x 18) against T (normalised targets, n x 3).
-// Gradients are accumulated into G (zeroed by the caller), the loss sum into
-// *loss. Each wavefront keeps its weight-gradient blocks in registers across
-// its tiles and adds them once.
+// features, row-major n x 18) against T (normalised targets, n x 3). Each
+// wavefront keeps its weight-gradient blocks in registers across its tiles
+// and writes them to its row of G (waves x kParams partial sums, reduced by
+// k_mlp_reduce); the loss sum is added into *loss.
 __global__ __launch_bounds__(kBlock) void k_mlp_train(int n, const float* __restrict__ X, const float* __restrict__ T,
                                                     const int32_t* __restrict__ idx, const float* __restrict__ P,
                                                     float* __restrict__ G, double* __restrict__ loss) {
@@ -306,34 +327,59 @@ __global__ __launch_bounds__(kBlock) void k_mlp_train(int n, const float* __rest
 #pragma unroll
     for (int i = 0; i < 4; ++i) gb3[i] += __shfl_xor(gb3[i], off);
   }
-  // D layout of every gradient block: row 4 g + i, column c
+  // this wavefront's gradient into its own row of the partial sums (every
+  // parameter written, padding as zeros); k_mlp_reduce adds the rows in
+  // wavefront order. D layout of every gradient block: row 4 g + i, column c
+  float* Gw = G + (int64_t)wave * kParams;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int o = 16 * mb + 4 * g + i;
 #pragma unroll
-      for (int b = 0; b < 2; ++b) atomicAdd(G + oW1 + o * kInPad + 16 * b + c, gw1[mb][b][i]);
+      for (int b = 0; b < 2; ++b) Gw[oW1 + o * kInPad + 16 * b + c] = gw1[mb][b][i];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) atomicAdd(G + oW2 + o * kHid + 16 * b + c, gw2[mb][b][i]);
+      for (int b = 0; b < 4; ++b) Gw[oW2 + o * kHid + 16 * b + c] = gw2[mb][b][i];
       if (c == 0) {
-        atomicAdd(G + oB1 + o, gb1[mb][i]);
-        atomicAdd(G + oB2 + o, gb2[mb][i]);
+        Gw[oB1 + o] = gb1[mb][i];
+        Gw[oB2 + o] = gb2[mb][i];
       }
     }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int o = 4 * g + i;
-    if (o < kOut) {
 #pragma unroll
-      for (int b = 0; b < 4; ++b) atomicAdd(G + oW3 + o * kHid + 16 * b + c, gw3[b][i]);
-      if (c == 0) atomicAdd(G + oB3 + o, gb3[i]);
-    }
+    for (int b = 0; b < 4; ++b) Gw[oW3 + o * kHid + 16 * b + c] = gw3[b][i];
+    if (c == 0) Gw[oB3 + o] = gb3[i];
   }
   // loss: wavefront sum, one atomic
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
   if (l == 0) atomicAdd(loss, (double)lsum);
+}
+
+// Gradient = sum of the wavefronts' rows in a fixed order (no atomics: the
+// same batch gives the same gradient bit for bit), in two passes: group y
+// of kReduceGroups adds the rows w = y, y + groups, ... (coalesced over the
+// parameter index), then the groups' sums are added in group order.
+constexpr int kReduceGroups = 64;
+
+__global__ __launch_bounds__(kBlock) void k_mlp_reduce_rows(int waves, const float* __restrict__ partial,
+                                                          float* __restrict__ sums) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (t >= kParams) return;
+  float acc = 0.0f;
+  for (int w = y; w < waves; w += kReduceGroups) acc += partial[(int64_t)w * kParams + t];
+  sums[(int64_t)y * kParams + t] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mlp_reduce_groups(const float* __restrict__ sums, float* __restrict__ G) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= kParams) return;
+  float acc = 0.0f;
+  for (int y = 0; y < kReduceGroups; ++y) acc += sums[(int64_t)y * kParams + t];
+  G[t] = acc;
 }
 
 // Adam (D. P. Kingma and J. Ba, "Adam: a method for stochastic
@@ -414,8 +460,10 @@ __global__ __launch_bounds__(kBlock) void k_mlp_forward(int n, const float* __re
 
 unsigned mlp_grid(int n) {
   const int tiles = (n + 15) / 16;
-  // up to 4 tiles per wavefront; at most 1024 workgroups
-  return (unsigned)std::max(1, std::min(1024, (tiles + 15) / 16));
+  // one tile per wavefront (4 per workgroup) up to 1024 workgroups (4 per
+  // CU), then several per wavefront: a batch of 16,384 rows spreads over
+  // all 256 CUs
+  return (unsigned)std::max(1, std::min(1024, (tiles + 3) / 4));
 }
 
 }  // namespace
@@ -425,14 +473,30 @@ using namespace fsi;
 
 extern "C" {
 
+int fsi_mlp_train_workspace(int32_t n, int64_t* floats) {
+  FSI_REQUIRE(n >= 0 && floats, "mlp_train_workspace: bad arguments");
+  *floats = ((int64_t)mlp_grid(std::max(n, 1)) * (kBlock / 64) + kReduceGroups) * kParams;
+  return 0;
+}
+
 int fsi_mlp_train_step(int32_t n, const float* X, const float* T, const int32_t* idx, const float* params,
-                       float* grads, double* loss_sum, void* stream) {
+                       float* grads, double* loss_sum, float* workspace, int64_t workspace_floats, void* stream) {
   FSI_REQUIRE(n >= 0, "mlp_train_step: negative batch size");
-  FSI_REQUIRE(X && T && params && grads && loss_sum, "mlp_train_step: null buffer");
-  if (n == 0) return 0;
-  hipLaunchKernelGGL(k_mlp_train, dim3(mlp_grid(n)), dim3(kBlock), 0, (hipStream_t)stream, n, X, T, idx, params, grads,
-                     loss_sum);
+  FSI_REQUIRE(X && T && params && grads && loss_sum && workspace, "mlp_train_step: null buffer");
+  const unsigned blocks = mlp_grid(std::max(n, 1));
+  const int waves = (int)blocks * (kBlock / 64);
+  const int64_t need = ((int64_t)waves + kReduceGroups) * kParams;
+  FSI_REQUIRE(workspace_floats >= need, "mlp_train_step: workspace of %lld floats, need %lld",
+              (long long)workspace_floats, (long long)need);
+  hipStream_t st = (hipStream_t)stream;
+  float* sums = workspace + (int64_t)waves * kParams;
+  hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(kBlock), 0, st, n, X, T, idx, params, workspace, loss_sum);
   FSI_TRY_LAUNCH("mlp_train_step");
+  hipLaunchKernelGGL(k_mlp_reduce_rows, dim3(blocks_for(kParams), kReduceGroups), dim3(kBlock), 0, st, waves,
+                     workspace, sums);
+  FSI_TRY_LAUNCH("mlp_reduce_rows");
+  hipLaunchKernelGGL(k_mlp_reduce_groups, dim3(blocks_for(kParams)), dim3(kBlock), 0, st, sums, grads);
+  FSI_TRY_LAUNCH("mlp_reduce_groups");
   return 0;
 }
 

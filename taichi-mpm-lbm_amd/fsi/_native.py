@@ -119,7 +119,7 @@ _SIGNATURES = {
     "fsi_abi_version": [],
     "fsi_fill_zero": [ctypes.c_int64, _vp, _vp],
     "fsi_closure_features": [_vp, _vp, _vp, _vp, _vp],
-    "fsi_mlp_train_step": [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "fsi_mlp_train_step": [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp],
     "fsi_mlp_adam": [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                      ctypes.c_float, _vp],
     "fsi_mlp_forward": [ctypes.c_int32, _vp, _vp, _vp, _vp],
@@ -150,6 +150,8 @@ def load_library() -> ctypes.CDLL:
             fn.restype = ctypes.c_int
         lib.fsi_mpm_sort_workspace.argtypes = [_vp, ctypes.POINTER(ctypes.c_int64)]
         lib.fsi_mpm_sort_workspace.restype = ctypes.c_int
+        lib.fsi_mlp_train_workspace.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]
+        lib.fsi_mlp_train_workspace.restype = ctypes.c_int
         lib.fsi_last_error.argtypes = []
         lib.fsi_last_error.restype = ctypes.c_char_p
         _lib = lib

@@ -74,6 +74,7 @@ class ClosureMLP:
         self.m = torch.zeros_like(self.params)
         self.v = torch.zeros_like(self.params)
         self._loss = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self._ws: torch.Tensor | None = None
         self.step_count = 0
 
     def forward(self, X: torch.Tensor) -> torch.Tensor:
@@ -89,10 +90,14 @@ class ClosureMLP:
         X = _rows(X, IN, self.device)
         T = _rows(T, OUT, self.device)
         n = X.shape[0] if idx is None else int(idx.numel())
-        nat.fill_zero(self.grads)
+        need = ctypes.c_int64(0)
+        if nat.load_library().fsi_mlp_train_workspace(n, ctypes.byref(need)) != 0:
+            raise RuntimeError(nat.load_library().fsi_last_error().decode(errors="replace"))
+        if self._ws is None or self._ws.numel() < need.value:
+            self._ws = torch.empty(int(need.value), dtype=torch.float32, device=self.device)
         nat.fill_zero(self._loss)
         nat.call("fsi_mlp_train_step", n, nat.ptr(X), nat.ptr(T), nat.ptr(idx), nat.ptr(self.params),
-                 nat.ptr(self.grads), nat.ptr(self._loss))
+                 nat.ptr(self.grads), nat.ptr(self._loss), nat.ptr(self._ws), self._ws.numel())
         return float(self._loss.item()) / (3.0 * max(n, 1))
 
     def adam(self, lr: float = 1.0e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1.0e-8) -> None:
