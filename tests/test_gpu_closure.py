@@ -187,3 +187,16 @@ def test_trained_closure_gains_over_the_plain_coarse_force():
         print(f"held-out scene {s}: rms force difference plain {rms_plain:.3e}, with closure {rms_closure:.3e}")
     print(f"training samples {len(X)}, loss {hist[0]:.3e} -> {hist[-1]:.3e}, held-out ratios {gains}")
     assert max(gains) < 0.9
+
+
+def test_gradient_is_deterministic():
+    """Per-wavefront partial sums added in a fixed order: the same batch
+    gives the same gradient bit for bit (no float atomics)."""
+    from fsi.closure import ClosureMLP
+
+    X, T = _data(20000, 9)
+    mlp = ClosureMLP(mo.init_params(2))
+    mlp.gradient(X, T)
+    g1 = mlp.grads.cpu().numpy().copy()
+    mlp.gradient(X, T)
+    np.testing.assert_array_equal(mlp.grads.cpu().numpy(), g1)
