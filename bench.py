@@ -180,46 +180,38 @@ def time_coupled_large(steps, warmup, storage="fp32"):
             "particle_substeps_per_s": round(len(pos) * n_sub * steps / dt, 1)}
 
 
-def time_closure(train_scenes=8, train_steps=40, eval_steps=60, speed_steps=2):
-    """Learned closure: trains on coarse/2x-refined pairs of single-ellipsoid
-    scenes (48x24x24 coarse), compares drag and centroid displacement of a
-    held-out scene (plain coarse, coarse + closure, refined), then times the
-    coarse run with FP16 populations and the closure against the refined
-    FP32 run on a 64-sphere channel (192x96x96 coarse, 384x192x192 refined)."""
+def time_closure(train_scenes=8, pack_scenes=4, train_steps=40, eval_steps=60, large_eval_steps=40, speed_steps=2):
+    """Learned closure: trains on coarse/2x-refined pairs (single-ellipsoid
+    and four-sphere channels, 48x24x24 coarse), then on a held-out ellipsoid
+    scene and on the 64-sphere channel (192x96x96 coarse, FP16 populations;
+    384x192x192 refined, FP32) compares drag and centroid displacement of
+    the plain coarse run and of the coarse run with the closure against the
+    refined run, and times the coarse run with the closure against the
+    refined run."""
     import numpy as np
     import torch
 
-    from fsi.closure import (ClosureScene, drag_and_displacement, pair_config, pair_simulation,
-                             refine_particles, scene_pair_data, train_closure)
-    from fsi.scenes import sphere_pack
+    from fsi.closure import ClosureScene, PackScene, accuracy_vs_refined, scene_pair_data, train_closure
 
     t0 = time.perf_counter()
-    data = [scene_pair_data(ClosureScene.random(s, shape=(48, 24, 24), axes=(3.0, 5.0)), train_steps, every=2,
-                            warmup=6) for s in range(300, 300 + train_scenes)]
+    scenes = ([ClosureScene.random(s, shape=(48, 24, 24), axes=(3.0, 5.0)) for s in range(300, 300 + train_scenes)]
+              + [PackScene.random(s, shape=(48, 24, 24), count=4, radii=(2.5, 5.0))
+                 for s in range(500, 500 + pack_scenes)])
+    data = [scene_pair_data(sc, train_steps, every=2, warmup=6) for sc in scenes]
     X = np.concatenate([d[0] for d in data])
     T = np.concatenate([d[1] for d in data])
     t1 = time.perf_counter()
     closure, hist = train_closure(X, T, steps=1500, batch=16384, lr=2e-3, seed=1)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    scene = ClosureScene.random(400, shape=(48, 24, 24), axes=(3.0, 5.0))
-    acc = {}
-    for name, kw in (("plain", {}), ("closure", {"closure": closure}), ("fine", {"refined": True})):
-        acc[name] = drag_and_displacement(scene, eval_steps, **kw)
-    fine = acc["fine"]
-
-    def err(r, key):
-        return round(abs(r[key] / fine[key] - 1.0), 4)
-
-    accuracy = {name: {"drag_rel_err": err(acc[name], "drag_x"),
-                       "displacement_rel_err": err(acc[name], "displacement_norm")} for name in ("plain", "closure")}
-    # speed at the large-scene scale
-    shape = (192, 96, 96)
-    parents, _ = sphere_pack(shape, 64, 2.5, 7.0, spacing=0.5, seed=5, margin=4.0, gap=2.0)
-    children = refine_particles(parents)
-    coarse = pair_simulation(pair_config(shape, False, len(parents), 1.0e3, 0.3, 0.04, storage="fp16"), parents)
+    held_out = accuracy_vs_refined(ClosureScene.random(400, shape=(48, 24, 24), axes=(3.0, 5.0)), eval_steps,
+                                   closure)
+    large = PackScene(shape=(192, 96, 96), count=64, radii=(2.5, 7.0), youngs_modulus=1.0e3, poisson_ratio=0.3,
+                      inflow=0.04, margin=4.0, gap=2.0, seed=5)
+    large_acc = accuracy_vs_refined(large, large_eval_steps, closure, storage="fp16")
+    coarse = large.simulation(False, "fp16")
     coarse.coupler.set_closure(closure)
-    fine_sim = pair_simulation(pair_config(shape, True, len(children), 1.0e3, 0.3, 0.04), children)
+    fine_sim = large.simulation(True)
 
     def timed(sim, n):
         sim.step()
@@ -232,13 +224,20 @@ def time_closure(train_scenes=8, train_steps=40, eval_steps=60, speed_steps=2):
 
     tc = timed(coarse, speed_steps)
     tf = timed(fine_sim, 2 * speed_steps) * 2.0
-    return {"train": {"scenes": train_scenes, "samples": int(len(X)), "adam_steps": 1500,
+    return {"train": {"scenes": len(scenes), "samples": int(len(X)), "adam_steps": 1500,
                       "data_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
                       "loss_first": round(hist[0], 4), "loss_last": round(hist[-1], 4)},
-            "accuracy_vs_refined": {"scene": "ellipsoid seed 400, 48x24x24 coarse, 60 coarse steps", **accuracy},
+            "accuracy_vs_refined": {
+                "metric": "drag = mean fluid-on-solid x-force per step (substep-averaged coupling force); "
+                          "displacement = per-body centroid displacement vector error",
+                "ellipsoid": {"scene": f"ellipsoid seed 400, 48x24x24 coarse FP32, {eval_steps} coarse steps",
+                              **held_out},
+                "spheres64": {"scene": f"64 spheres, 192x96x96 coarse FP16 vs 384x192x192 FP32, "
+                                       f"{large_eval_steps} coarse steps", **large_acc}},
             "speed": {"workload": "64 spheres r 2.5-7 (coarse cells), 192x96x96 FP16 lattice + closure vs "
                                   "384x192x192 FP32 refined, 112 substeps",
-                      "coarse_particles": int(len(parents)), "fine_particles": int(len(children)),
+                      "coarse_particles": int(coarse.mpm.num_particles),
+                      "fine_particles": int(fine_sim.mpm.num_particles),
                       "coarse_closure_ms_per_step": round(1e3 * tc, 3),
                       "fine_ms_per_coarse_step": round(1e3 * tf, 3), "speedup": round(tf / tc, 2),
                       "closure_fallbacks": coarse.coupler.closure_fallback_count()}}

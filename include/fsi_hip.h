@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FSI_ABI_VERSION 1
+#define FSI_ABI_VERSION 2
 
 enum {
   FSI_OK = 0,
@@ -209,6 +209,13 @@ typedef struct fsi_coupling_buffers {
   int32_t* counter_rows;   /* optional scratch for the fused substeps: 8 int32 per
                               substep (the last substep's row is copied to counters) */
   int32_t counter_rows_cap; /* substeps it holds */
+  void* force_mean;        /* optional 3*cap real (SoA): when set, every coupled substep
+                              adds its particle's coupling force times the substep's
+                              fraction of the lattice step, so after a step that began
+                              with it zeroed it holds the step-averaged coupling force
+                              per particle slot. No counterpart in the reference (which
+                              keeps only the last substep's force, fsi/coupling.py:389):
+                              the learned closure's training targets use it. */
 } fsi_coupling_buffers;
 
 /* Zero the grid (fsi/mpm3d.py:205-209, clear_grid). */

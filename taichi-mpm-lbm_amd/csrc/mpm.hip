@@ -57,6 +57,7 @@ template <typename R> struct CplK {
   R force_limit;
   int contact;
   R contact_thr, contact_damp;
+  R* force_mean;  // optional step-averaged coupling force accumulator (3 x cap)
 };
 
 template <typename R> CplK<R> make_cpl(const fsi_coupling_params* c) {
@@ -832,6 +833,10 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     if (live) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) pforce[a * cap + p] = pf[a];
+      if (cp.force_mean) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) cp.force_mean[a * cap + p] += pf[a] * cp.dt_ratio;
+      }
     }
   } else {
     if (clear_force) {
@@ -1480,6 +1485,7 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
   MpmK<R> m = make_mpm<R>(p);
   int32_t* ctrs = counters_override ? counters_override : (cb ? cb->counters : nullptr);
   CplK<R> cp = make_cpl<R>(c);
+  cp.force_mean = cb ? (R*)cb->force_mean : nullptr;
   const bool couple = c && c->enabled;
   if (couple && zero_counters) {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 0, 5);
@@ -1621,6 +1627,7 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
   const fsi_coupling_params* lc = c ? &local : nullptr;
   MpmK<R> m = make_mpm<R>(p);
   CplK<R> cp = make_cpl<R>(lc);
+  cp.force_mean = cb ? (R*)cb->force_mean : nullptr;
   const bool couple = lc && lc->enabled;
   const bool contact = lc && lc->contact_enabled;
   const int32_t* chunks = (b->chunks && p->num_chunks > 0) ? b->chunks : nullptr;

@@ -84,7 +84,7 @@ class CouplingParams(ctypes.Structure):
 class CouplingBuffers(ctypes.Structure):
     _fields_ = [("solid", _vp), ("lbm_vel", _vp), ("cforce", _vp), ("valid_weight", _vp),
                 ("coupling_mask", _vp), ("contact_mask", _vp), ("counters", _vp), ("closure_force", _vp),
-                ("counter_rows", _vp), ("counter_rows_cap", ctypes.c_int32)]
+                ("counter_rows", _vp), ("counter_rows_cap", ctypes.c_int32), ("force_mean", _vp)]
 
 
 class ForceAssembly(ctypes.Structure):

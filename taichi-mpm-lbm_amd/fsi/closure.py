@@ -12,7 +12,8 @@ elastic moduli and densities are unchanged, and a force in coarse lattice
 units is a quarter of the same force in fine lattice units. Each coarse
 particle is the parent of the 8 fine particles that fill its volume, so the
 target for a coarse particle is (sum of its children's fine forces) / 4 minus
-its own coarse force at the same physical time.
+its own coarse force, both averaged over the substeps of the same physical
+interval (one coarse step, two fine steps).
 
 Everything on the device runs in the library's HIP kernels:
 ``fsi_closure_features`` (features), ``fsi_mlp_train_step`` (fused forward,
@@ -26,6 +27,7 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
+import functools
 import math
 from pathlib import Path
 
@@ -258,7 +260,9 @@ def pair_config(shape, refined: bool, n_particles: int, youngs_modulus: float, p
     """SimulationConfig of a channel (velocity inflow on x-, pressure outflow
     on x+, fluid starting at the inflow velocity) for the coarse run or its
     2x refinement (acoustic scaling: viscosity x 2, penalty rate / 2, same
-    moduli, density and lattice velocities)."""
+    moduli, density and lattice velocities; the MPM boundary layer keeps its
+    physical width, 2 coarse cells = 4 fine cells, so a body reaches it at
+    the same time in both runs)."""
     from .config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
 
     f = 2 if refined else 1
@@ -269,7 +273,7 @@ def pair_config(shape, refined: bool, n_particles: int, youngs_modulus: float, p
                     x_right=BoundaryConfig("pressure", rho=1.0))
     mpm = MPMConfig(nx=nx, ny=ny, nz=nz, max_particles=int(n_particles), dx=1.0, dt=1.0 / substeps,
                     density=density, youngs_modulus=youngs_modulus, poisson_ratio=poisson_ratio,
-                    gravity=(0.0, 0.0, 0.0), boundary_width=2)
+                    gravity=(0.0, 0.0, 0.0), boundary_width=2 * f)
     cpl = CouplingConfig(gamma=gamma / f, mpm_substeps_per_lbm_step=substeps)
     return SimulationConfig(num_steps=1, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
 
@@ -345,6 +349,9 @@ class ClosureScene:
         parents, children = self.particles()
         return pair_simulation(self.config(refined, storage), children if refined else parents)
 
+    def bodies(self) -> list[tuple[int, int]]:
+        return [(0, len(self.particles()[0]))]
+
 
 class _FeatureProbe:
     """Feature rows of a coupler's particles in input order (host)."""
@@ -362,46 +369,170 @@ class _FeatureProbe:
         return mpm._input_order(self.X[: mpm.num_particles].cpu().numpy().copy())
 
 
-def scene_pair_data(scene: ClosureScene, steps: int, every: int = 4, warmup: int = 8) -> tuple[np.ndarray, np.ndarray]:
+def scene_pair_data(scene, steps: int, every: int = 4, warmup: int = 8) -> tuple[np.ndarray, np.ndarray]:
     """Runs the coarse and refined simulations of a scene side by side and
-    returns (features, targets) of the coarse particles at coarse steps
-    warmup, warmup + every, ... < steps."""
+    returns (features, targets) of the coarse particles for coarse steps
+    warmup, warmup + every, ... < steps: the features of the state at the
+    start of the step and the difference of the step-averaged coupling
+    forces over that step (the refined run's two steps: the mean of their
+    step averages, children summed, in coarse units). The correction is
+    applied on every substep of a coarse step, so its target is the
+    difference of the forces averaged over the step, not of the last
+    substep's forces (a point sample of a force that relaxes within a step
+    twice as long as a fine one)."""
     coarse = scene.simulation(False)
     fine = scene.simulation(True)
+    coarse.coupler.record_step_force()
+    fine.coupler.record_step_force()
     probe = _FeatureProbe(coarse.coupler)
     Xs, Ts = [], []
     for k in range(steps):
-        if k >= warmup and (k - warmup) % every == 0:
+        take = k >= warmup and (k - warmup) % every == 0
+        if take:
             X = probe.features()
-            fc = coarse.mpm.particle_forces_numpy().astype(np.float64)
-            ff = fine.mpm.particle_forces_numpy().astype(np.float64).reshape(-1, 8, 3).sum(1) / 4.0
             active = coarse.mpm.active_numpy() != 0
+        coarse.step()
+        ff = 0.0
+        for _ in range(2):
+            fine.step()
+            if take:
+                ff = ff + fine.coupler.step_force_numpy().astype(np.float64)
+        if take:
+            fc = coarse.coupler.step_force_numpy().astype(np.float64)
+            ff = ff.reshape(-1, 8, 3).sum(1) / (2.0 * 4.0)
             Xs.append(X[active])
             Ts.append((ff - fc)[active])
-        coarse.step()
-        fine.step()
-        fine.step()
     return np.concatenate(Xs).astype(np.float32), np.concatenate(Ts).astype(np.float32)
 
 
-def drag_and_displacement(scene: ClosureScene, steps: int, closure: LearnedClosure | None = None,
-                          refined: bool = False) -> dict:
-    """Mean x-force on the solid over the second half of the run (coarse
-    lattice units: fine forces / 4) and the centroid displacement at the end
-    (coarse cells), of the coarse run (optionally with the closure) or the
-    refined run (twice the steps)."""
-    sim = scene.simulation(refined)
-    if closure is not None:
-        sim.coupler.set_closure(closure)
+@dataclasses.dataclass(frozen=True)
+class PackScene:
+    """Several elastic spheres in a channel with inflow (the coarse
+    description; `refined` doubles it), the multi-body counterpart of
+    ``ClosureScene``."""
+
+    shape: tuple[int, int, int] = (48, 24, 24)
+    count: int = 4
+    radii: tuple[float, float] = (2.5, 5.0)
+    youngs_modulus: float = 1.0e3
+    poisson_ratio: float = 0.3
+    inflow: float = 0.04
+    viscosity: float = 0.02
+    gamma: float = 0.5
+    substeps: int = 112
+    density: float = 1.2
+    margin: float = 3.0
+    gap: float = 1.5
+    upstream: float = 1.0
+    seed: int = 0
+
+    @classmethod
+    def random(cls, seed: int, shape=(48, 24, 24), count=4, radii=(2.5, 5.0), modulus=(2.0e2, 2.0e3),
+               poisson=(0.2, 0.4), inflow=(0.01, 0.05), upstream=0.7) -> "PackScene":
+        """E log-uniform in `modulus`, nu U[poisson], inflow U[inflow]; the
+        spheres are packed into the first `upstream` fraction of the
+        channel, so that none drifts into the outflow's boundary layer."""
+        rng = np.random.default_rng(seed + 104729)
+        return cls(shape=tuple(shape), count=count, radii=tuple(radii), upstream=float(upstream),
+                   youngs_modulus=float(math.exp(rng.uniform(math.log(modulus[0]), math.log(modulus[1])))),
+                   poisson_ratio=float(rng.uniform(*poisson)), inflow=float(rng.uniform(*inflow)), seed=seed)
+
+    def _pack(self):
+        return _packed_spheres(self)
+
+    def particles(self) -> tuple[np.ndarray, np.ndarray]:
+        parents, _ = self._pack()
+        return parents, refine_particles(parents)
+
+    def bodies(self) -> list[tuple[int, int]]:
+        """Coarse particle range [start, stop) of each sphere."""
+        counts = self._pack()[1]["counts"]
+        ends = np.cumsum(counts)
+        return [(int(e - c), int(e)) for c, e in zip(counts, ends)]
+
+    def config(self, refined: bool = False, storage: str = "fp32"):
+        npart = len(self.particles()[1 if refined else 0])
+        return pair_config(self.shape, refined, npart, self.youngs_modulus, self.poisson_ratio, self.inflow,
+                           self.viscosity, self.gamma, self.substeps, self.density, storage)
+
+    def simulation(self, refined: bool = False, storage: str = "fp32"):
+        parents, children = self.particles()
+        return pair_simulation(self.config(refined, storage), children if refined else parents)
+
+
+@functools.lru_cache(maxsize=8)
+def _packed_spheres(scene: PackScene):
+    from .scenes import sphere_pack
+
+    box = (int(round(scene.shape[0] * scene.upstream)),) + tuple(scene.shape[1:])
+    return sphere_pack(box, scene.count, scene.radii[0], scene.radii[1], spacing=0.5, seed=scene.seed,
+                       margin=scene.margin, gap=scene.gap)
+
+
+def run_observables(sim, steps: int, bodies, refined: bool = False) -> dict:
+    """Runs `steps` coarse steps (twice as many fine steps when `refined`)
+    and returns, in coarse lattice units, the drag per coarse step and each
+    body's centroid displacement.
+
+    The drag is the x-force of the fluid on the solids: the step's coupling
+    force summed over the lattice (an FP64 device sum), sign flipped. It is
+    the substep-averaged force (the coupling buffer accumulates every
+    substep's spread force times its time fraction), so it is the momentum
+    the fluid gives the solids per unit time. The end-of-step particle force
+    is a point sample of a force that relaxes within the step, and a fine
+    step is half as long, so that sample does not compare across
+    resolutions. A fine force is a quarter of the same force in coarse
+    units; the fine drag of a coarse step is the mean of its two fine steps.
+    `bodies` are coarse particle ranges (children of coarse particle i are
+    fine particles 8 i .. 8 i + 7)."""
     f = 2 if refined else 1
-    x0 = sim.mpm.positions_numpy().astype(np.float64).mean(0) / f
-    drag = []
-    n = steps * f
-    for k in range(n):
-        sim.step()
-        if k >= n // 2:
-            drag.append(float(sim.mpm.particle_forces_numpy().astype(np.float64)[:, 0].sum()))
-    x1 = sim.mpm.positions_numpy().astype(np.float64).mean(0) / f
-    scale = 4.0 if refined else 1.0
-    return {"drag_x": float(np.mean(drag)) / scale, "displacement": (x1 - x0).tolist(),
-            "displacement_norm": float(np.linalg.norm(x1 - x0))}
+    rows = [(f ** 3 * a, f ** 3 * b) for a, b in bodies]
+
+    def centroids():
+        x = sim.mpm.positions_numpy().astype(np.float64) / f
+        return np.stack([x[a:b].mean(0) for a, b in rows])
+
+    c0 = centroids()
+    drag = np.zeros(steps)
+    for k in range(steps):
+        for _ in range(f):
+            sim.step()
+            drag[k] -= float(sim.lbm.lattice_sums(sim.coupler.cforce)[2])
+        drag[k] /= f * f * f
+    return {"drag_x": drag, "displacement": centroids() - c0,
+            "fallbacks": sim.coupler.closure_fallback_count() if sim.coupler.closure is not None else 0}
+
+
+def observable_errors(run: dict, fine: dict) -> dict:
+    """Relative errors of a coarse run against the refined run: mean drag
+    over the whole run and over its second half, and the displacement error
+    |d - d_fine| / |d_fine| (vectors) averaged over the bodies."""
+    d, df = run["drag_x"], fine["drag_x"]
+    h = len(d) // 2
+    disp = np.linalg.norm(run["displacement"] - fine["displacement"], axis=1) / np.maximum(
+        np.linalg.norm(fine["displacement"], axis=1), 1e-30)
+    return {"drag_rel_err": round(float(abs(d.mean() / df.mean() - 1.0)), 4),
+            "drag_rel_err_second_half": round(float(abs(d[h:].mean() / df[h:].mean() - 1.0)), 4),
+            "displacement_rel_err": round(float(disp.mean()), 4),
+            "displacement_rel_err_max": round(float(disp.max()), 4)}
+
+
+def accuracy_vs_refined(scene, steps: int, closure: LearnedClosure | None = None, storage: str = "fp32") -> dict:
+    """Drag and displacement errors of the coarse run, plain and with the
+    closure (coarse populations stored as `storage`), against the refined
+    FP32 run of the same scene; the closure run's fallback count (particle
+    evaluations outside the training envelope) and evaluations."""
+    bodies = scene.bodies()
+    fine = run_observables(scene.simulation(True), steps, bodies, refined=True)
+    out = {"fine_mean_drag_x": float(fine["drag_x"].mean())}
+    runs = [("plain", None)] + ([("closure", closure)] if closure is not None else [])
+    for name, cl in runs:
+        sim = scene.simulation(False, storage)
+        if cl is not None:
+            sim.coupler.set_closure(cl)
+        r = run_observables(sim, steps, bodies)
+        out[name] = observable_errors(r, fine)
+        if cl is not None:
+            out[name]["fallbacks"] = r["fallbacks"]
+            out[name]["evaluations"] = int(steps * sim.mpm.num_particles)
+    return out

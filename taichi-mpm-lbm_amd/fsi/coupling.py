@@ -80,6 +80,8 @@ class LBMMpmCoupler:
         # the penalty force, evaluated once per coupled step
         self.closure = None
         self.closure_force: torch.Tensor | None = None
+        # optional step-averaged coupling force per particle slot
+        self.force_mean: torch.Tensor | None = None
 
     def set_closure(self, closure) -> None:
         """Attach a ``fsi.closure.LearnedClosure`` (None detaches it). Each
@@ -92,14 +94,37 @@ class LBMMpmCoupler:
             self.closure_force = None
             self._cbuf.closure_force = None
             return
+        self._fallback_base = int(closure.fallbacks.item())
         cap = self.mpm.max_particles
         self.closure_force = torch.zeros(3 * cap, dtype=self.lbm.real_dtype, device=self.lbm.device)
         self._cbuf.closure_force = self.closure_force.data_ptr()
 
+    def record_step_force(self, enabled: bool = True) -> None:
+        """Keep the step-averaged coupling force of every particle (the mean
+        over the step's substeps, which the reference does not store: its
+        particle force is the last substep's, fsi/coupling.py:389); read it
+        with ``step_force_numpy``. Not part of the reference."""
+        if enabled:
+            self.force_mean = torch.zeros(3 * self.mpm.max_particles, dtype=self.lbm.real_dtype,
+                                          device=self.lbm.device)
+            self._cbuf.force_mean = self.force_mean.data_ptr()
+        else:
+            self.force_mean = None
+            self._cbuf.force_mean = None
+
+    def step_force_numpy(self) -> np.ndarray:
+        """(n, 3) step-averaged coupling force of the last step, in input
+        order (``record_step_force`` must be on)."""
+        if self.force_mean is None:
+            raise RuntimeError("record_step_force() is off")
+        n, cap = self.mpm.num_particles, self.mpm.max_particles
+        f = self.force_mean.view(3, cap)[:, :n].cpu().numpy().T.copy()
+        return self.mpm._input_order(f)
+
     def closure_fallback_count(self) -> int:
         """Particle evaluations that left the closure's training envelope (and
         kept the plain force) since the closure was attached."""
-        return 0 if self.closure is None else int(self.closure.fallbacks.item())
+        return 0 if self.closure is None else int(self.closure.fallbacks.item()) - self._fallback_base
 
     def _validate_solver_contract(self) -> None:
         # fsi/coupling.py:42-56
@@ -187,6 +212,8 @@ class LBMMpmCoupler:
         if self.closure is not None and self.enabled:
             # features read the occupancy of the last step: before the clear
             self.closure.apply(self, self.closure_force)
+        if self.force_mean is not None:
+            nat.fill_zero(self.force_mean)
         self.clear_coupling_fields()
         mpm_dt = lbm_dt_value / float(self.substeps)
         dt_ratio = mpm_dt / lbm_dt_value

@@ -200,3 +200,34 @@ def test_gradient_is_deterministic():
     g1 = mlp.grads.cpu().numpy().copy()
     mlp.gradient(X, T)
     np.testing.assert_array_equal(mlp.grads.cpu().numpy(), g1)
+
+
+def test_step_force_conserves_momentum_with_the_lattice():
+    """The step-averaged particle forces (the closure's training targets)
+    sum to minus the coupling force the same substeps spread to the fluid
+    (every node of a particle's support is fluid or dropped with its weight
+    renormalised, so the spread conserves momentum), and the last substep's
+    force is one of the averaged terms: the two stay within the relaxation
+    of one step."""
+    from fsi.closure import ClosureScene
+
+    scene = ClosureScene.random(13, shape=(24, 12, 12), axes=(2.5, 3.5))
+    sim = scene.simulation(False)
+    sim.coupler.record_step_force()
+    for _ in range(4):
+        sim.step()
+    fm = sim.coupler.step_force_numpy().astype(np.float64)
+    total = fm.sum(0)
+    fluid = -sim.coupler.total_fluid_coupling_force().astype(np.float64)
+    print(f"step-mean particle force {total}, fluid reaction {fluid}")
+    assert np.abs(total - fluid).max() <= 1e-4 * np.abs(fluid).max()
+    last = sim.mpm.particle_forces_numpy().astype(np.float64)
+    assert np.all(np.isfinite(fm)) and np.abs(fm).max() > 0
+    # gamma = 0.5 per step: within a step the slip relaxes by at most e^-0.5,
+    # so the last substep's force is between 0.5x and 1.5x the step mean on
+    # the body as a whole
+    assert 0.5 * abs(total[0]) <= abs(last[:, 0].sum()) <= 1.5 * abs(total[0])
+    sim.coupler.record_step_force(False)
+    sim.step()
+    with pytest.raises(RuntimeError):
+        sim.coupler.step_force_numpy()
