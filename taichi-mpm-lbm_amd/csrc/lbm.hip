@@ -202,6 +202,16 @@ template <typename T> __device__ __forceinline__ T ld_off(const T* base, uint32_
 template <typename T> __device__ __forceinline__ void st_off(T* base, uint32_t byte_off, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
+// Streaming store of a population: the next step reads it from HBM, not
+// from this step's caches (a 19-plane copy measured 3-4% faster with
+// nontemporal stores).
+template <typename T> __device__ __forceinline__ void st_stream(T* base, uint32_t byte_off, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off));
+}
+__device__ __forceinline__ void st_stream(__half* base, uint32_t byte_off, __half v) {
+  __builtin_nontemporal_store(__half_as_ushort(v),
+                              reinterpret_cast<unsigned short*>(reinterpret_cast<char*>(base) + byte_off));
+}
 
 struct Nbr {
   int64_t ox[3], oy[3], oz[3];  // index offsets for e = -1, 0, +1 with periodic wrap
@@ -454,7 +464,7 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
       const S v = solid ? solid_value<S>(q, c.rho0, c.scale) : PopStore<S>::store(d[q], c.scale);
-      st_off(pout, q * plane + own, v);
+      st_stream(pout, q * plane + own, v);
     }
     return;
   }
@@ -463,7 +473,7 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     const S v = solid ? solid_value<S>(q, c.rho0, c.scale) : PopStore<S>::store(d[q], c.scale);
-    st_off(pout, q * plane + own, v);
+    st_stream(pout, q * plane + own, v);
   }
 }
 
