@@ -70,6 +70,37 @@ def time_lattice(solver, steps, warmup, launch_samples):
     return wall, launch_ms
 
 
+def lattice_errors_vs_fp64(n, mask, fields, steps):
+    """Per-field error of the timed FP32 / FP16 lattice runs against the FP64
+    build of the same kernels after the same number of steps (the FP64 build
+    is pinned to the CPU oracle at 1e-10): max-norm relative error of u, its
+    L2-relative error, and the max-norm error of rho (absolute and relative
+    to max |rho - rho0| of the FP64 run), computed on the device."""
+    import torch
+
+    import fsi
+
+    s = fsi.LBMSolver3D(periodic_lbm_config(n, "fp64"))
+    s.initialize(mask)
+    for _ in range(steps):
+        s.step()
+    s._ensure_macro()
+    rho64, vel64 = s.rho, s.vel
+    drho64 = (rho64 - 1.0).abs().max().item()
+    out = {"steps": steps, "reference": "FP64 build of the same kernels, same mask and steps"}
+    for storage, (rho, vel) in fields.items():
+        dv = vel.double() - vel64
+        out[storage] = {
+            "u_rel_max": float(dv.abs().max().item() / vel64.abs().max().item()),
+            "u_rel_l2": float(dv.norm().item() / vel64.norm().item()),
+            "rho_abs_max": float((rho.double() - rho64).abs().max().item()),
+            "rho_rel_to_deviation": float((rho.double() - rho64).abs().max().item() / max(drho64, 1e-300)),
+        }
+    del s
+    torch.cuda.empty_cache()
+    return out
+
+
 def read_traffic(name):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     path = ROOT / "profiles" / name
@@ -292,6 +323,8 @@ def main():
     ap.add_argument("--no-coupled", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--no-error-check", action="store_true",
+                    help="skip the FP64 run that the FP32/FP16 lattice fields are compared with")
     ap.add_argument("--large-steps", type=int, default=3,
                     help="timed steps of the 256x128x128 coupled suspension (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -318,6 +351,8 @@ def main():
     n_fluid = int((mask == 0).sum())
 
     runs = {}
+    fields = {}
+    line_errors = None
     storages = ["fp32"] if args.no_fp16 else ["fp32", "fp16"]
     for storage in storages:
         s = fsi.LBMSolver3D(periodic_lbm_config(n, storage))
@@ -333,8 +368,13 @@ def main():
             "bytes": alg,
             "achieved_gbs": alg / (launch_ms * 1e-3) / 1e9,
         }
+        if not args.no_error_check:
+            s._ensure_macro()
+            fields[storage] = (s.rho.clone(), s.vel.clone())
         del s
         torch.cuda.empty_cache()
+    if fields:
+        line_errors = lattice_errors_vs_fp64(n, mask, fields, args.warmup + args.steps + args.launch_samples)
 
     r32 = runs["fp32"]
     traffic = read_traffic(f"lbm_step_{n}_fp32_pmc.txt")
@@ -381,6 +421,8 @@ def main():
             "speedup_vs_fp32": round(r16["mlups"] / r32["mlups"], 3),
             "traffic": read_traffic(f"lbm_step_{n}_fp16_pmc.txt"),
         }
+    if line_errors is not None:
+        line["error_vs_fp64"] = line_errors
     if not args.no_coupled:
         line["coupled"] = {"workload": "32^3 coupled penalty scene, FP32, 4096 particles, 112 substeps",
                            **time_coupled(args.coupled_steps, 2)}
