@@ -9,7 +9,8 @@ with rel = max|a - b| / max|b|:
 * FP32 build vs FP64 oracle: within twice the error the reference's own FP32
   arithmetic (the oracle's FP32 build, in the reference's operation order)
   makes against FP64, plus 1e-7 - i.e. as accurate as the reference.
-* FP16 shifted storage vs FP64 oracle: velocity rel <= 3e-3, density rel <= 1e-5.
+* FP16 shifted storage vs FP64 oracle: velocity rel <= 1e-3 (measured <= 3.8e-4 on these
+  scenarios), density rel <= 1e-5.
 """
 
 import dataclasses
@@ -112,7 +113,7 @@ def test_lbm_fp16_shifted_storage_error(name, oracle_runs):
     eu = rel_inf(got["u"], ref["u"])
     er = rel_inf(got["rho"], ref["rho"])
     print(f"{name} fp16: u rel {eu:.3e} rho rel {er:.3e}")
-    assert eu <= 3e-3 and er <= 1e-5, (name, eu, er)
+    assert eu <= 1e-3 and er <= 1e-5, (name, eu, er)
 
 
 def test_solid_cells_keep_initial_populations_and_zero_velocity():
