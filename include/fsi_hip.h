@@ -335,8 +335,8 @@ int fsi_reduce_particles(const fsi_mpm_params* p, const fsi_mpm_buffers* b, cons
 
 /* -------------------------------------------------- learned closure ---- */
 
-/* No counterpart in the reference (the north star's learned correction to
- * the penalty coupling force of fsi/coupling.py:333-406). An MLP
+/* No counterpart in the reference (a learned correction to the penalty
+ * coupling force of fsi/coupling.py:333-406). An MLP
  * 18 -> 64 -> 64 -> 3 with GELU, BF16 weights and activations, FP32
  * accumulation on the matrix cores. Parameters: one FP32 vector of
  * FSI_MLP_PARAMS floats, W1[64][32] (input columns 18..31 zero), b1[64],
