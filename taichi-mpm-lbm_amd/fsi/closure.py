@@ -469,10 +469,12 @@ def _packed_spheres(scene: PackScene):
                        margin=scene.margin, gap=scene.gap)
 
 
-def run_observables(sim, steps: int, bodies, refined: bool = False) -> dict:
+def run_observables(sim, steps: int, bodies, refined: bool = False, force_every: int = 0) -> dict:
     """Runs `steps` coarse steps (twice as many fine steps when `refined`)
-    and returns, in coarse lattice units, the drag per coarse step and each
-    body's centroid displacement.
+    and returns, in coarse lattice units, the drag per coarse step, each
+    body's centroid displacement and, every `force_every` coarse steps, the
+    step-averaged coupling force of every coarse particle (for the refined
+    run: its 8 children summed, mean of the two fine steps).
 
     The drag is the x-force of the fluid on the solids: the step's coupling
     force summed over the lattice (an FP64 device sum), sign flipped. It is
@@ -487,6 +489,8 @@ def run_observables(sim, steps: int, bodies, refined: bool = False) -> dict:
     fine particles 8 i .. 8 i + 7)."""
     f = 2 if refined else 1
     rows = [(f ** 3 * a, f ** 3 * b) for a, b in bodies]
+    if force_every:
+        sim.coupler.record_step_force()
 
     def centroids():
         x = sim.mpm.positions_numpy().astype(np.float64) / f
@@ -494,43 +498,61 @@ def run_observables(sim, steps: int, bodies, refined: bool = False) -> dict:
 
     c0 = centroids()
     drag = np.zeros(steps)
+    forces = []
     for k in range(steps):
+        take = force_every and (k + 1) % force_every == 0
+        pf = 0.0
         for _ in range(f):
             sim.step()
             drag[k] -= float(sim.lbm.lattice_sums(sim.coupler.cforce)[2])
+            if take:
+                pf = pf + sim.coupler.step_force_numpy().astype(np.float64)
         drag[k] /= f * f * f
-    return {"drag_x": drag, "displacement": centroids() - c0,
+        if take:
+            forces.append(pf.reshape(-1, f ** 3, 3).sum(1) / (f * f * f) if refined else pf)
+    return {"drag_x": drag, "displacement": centroids() - c0, "particle_forces": forces,
             "fallbacks": sim.coupler.closure_fallback_count() if sim.coupler.closure is not None else 0}
 
 
 def observable_errors(run: dict, fine: dict) -> dict:
     """Relative errors of a coarse run against the refined run: mean drag
-    over the whole run and over its second half, and the displacement error
-    |d - d_fine| / |d_fine| (vectors) averaged over the bodies."""
+    over the whole run and over its second half, the displacement error
+    |d - d_fine| / |d_fine| (vectors) averaged over the bodies and, when
+    both runs sampled them, the RMS per-particle step-averaged force error
+    relative to the RMS refined force (the quantity the closure corrects,
+    here with the closure in the loop)."""
     d, df = run["drag_x"], fine["drag_x"]
     h = len(d) // 2
     disp = np.linalg.norm(run["displacement"] - fine["displacement"], axis=1) / np.maximum(
         np.linalg.norm(fine["displacement"], axis=1), 1e-30)
-    return {"drag_rel_err": round(float(abs(d.mean() / df.mean() - 1.0)), 4),
-            "drag_rel_err_second_half": round(float(abs(d[h:].mean() / df[h:].mean() - 1.0)), 4),
-            "displacement_rel_err": round(float(disp.mean()), 4),
-            "displacement_rel_err_max": round(float(disp.max()), 4)}
+    out = {"drag_rel_err": round(float(abs(d.mean() / df.mean() - 1.0)), 4),
+           "drag_rel_err_second_half": round(float(abs(d[h:].mean() / df[h:].mean() - 1.0)), 4),
+           "displacement_rel_err": round(float(disp.mean()), 4),
+           "displacement_rel_err_max": round(float(disp.max()), 4)}
+    pa, pb = run.get("particle_forces") or [], fine.get("particle_forces") or []
+    if pa and len(pa) == len(pb):
+        num = sum(float(((a - b) ** 2).sum()) for a, b in zip(pa, pb))
+        den = sum(float((b ** 2).sum()) for b in pb)
+        out["particle_force_rel_rms"] = round(float(np.sqrt(num / max(den, 1e-300))), 4)
+    return out
 
 
-def accuracy_vs_refined(scene, steps: int, closure: LearnedClosure | None = None, storage: str = "fp32") -> dict:
-    """Drag and displacement errors of the coarse run, plain and with the
-    closure (coarse populations stored as `storage`), against the refined
-    FP32 run of the same scene; the closure run's fallback count (particle
+def accuracy_vs_refined(scene, steps: int, closure: LearnedClosure | None = None, storage: str = "fp32",
+                        force_every: int = 5) -> dict:
+    """Drag, displacement and per-particle force errors of the coarse run,
+    plain and with the closure (coarse populations stored as `storage`),
+    against the refined FP32 run of the same scene (forces sampled every
+    `force_every` coarse steps); the closure run's fallback count (particle
     evaluations outside the training envelope) and evaluations."""
     bodies = scene.bodies()
-    fine = run_observables(scene.simulation(True), steps, bodies, refined=True)
+    fine = run_observables(scene.simulation(True), steps, bodies, refined=True, force_every=force_every)
     out = {"fine_mean_drag_x": float(fine["drag_x"].mean())}
     runs = [("plain", None)] + ([("closure", closure)] if closure is not None else [])
     for name, cl in runs:
         sim = scene.simulation(False, storage)
         if cl is not None:
             sim.coupler.set_closure(cl)
-        r = run_observables(sim, steps, bodies)
+        r = run_observables(sim, steps, bodies, force_every=force_every)
         out[name] = observable_errors(r, fine)
         if cl is not None:
             out[name]["fallbacks"] = r["fallbacks"]
