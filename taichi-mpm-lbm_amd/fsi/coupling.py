@@ -113,7 +113,8 @@ class LBMMpmCoupler:
             self._cbuf.force_mean = None
 
     def step_force_numpy(self) -> np.ndarray:
-        """(n, 3) step-averaged coupling force of the last step, in input
+        """(n, 3) step-averaged coupling force of the last step (after
+        ``compute_coupling_forces``: its force times dt_ratio), in input
         order (``record_step_force`` must be on)."""
         if self.force_mean is None:
             raise RuntimeError("record_step_force() is off")
@@ -247,6 +248,8 @@ class LBMMpmCoupler:
         self.lbm._ensure_macro()
         self.clear_coupling_fields()
         self.mpm.clear_particle_force()
+        if self.force_mean is not None:
+            nat.fill_zero(self.force_mean)
         if not self.enabled:
             return
         mp = self.mpm.params(self.mpm.dt_default)
