@@ -143,6 +143,12 @@ typedef struct fsi_mpm_params {
   double clamp_lower, clamp_upper[3];
   int32_t num_chunks;      /* entries of fsi_mpm_buffers.chunks (0: P2G takes the
                               particles in slot order, 256 per workgroup) */
+  int32_t split_max_chunks; /* FP32 with a work list of at most this many chunks:
+                              each particle gets three wavefronts, one per x plane
+                              of its 27-node stencil, in the P2G/G2P passes of
+                              fsi_mpm_substeps (small particle sets, whose passes
+                              are bound by one wavefront's instruction latency);
+                              0 disables */
 } fsi_mpm_params;
 
 typedef struct fsi_mpm_buffers {

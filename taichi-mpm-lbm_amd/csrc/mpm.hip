@@ -409,6 +409,15 @@ __global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf
 // k_p2g): 2 resident workgroups per CU x 256 CUs.
 constexpr int kWideGridLimit = 512;
 
+// Plane split (PL = 3): a workgroup takes 64 particle slots (a quarter of a
+// work-list chunk) with three wavefronts, one per x plane of the stencil, so
+// the passes of a small particle set spread over four times as many CUs and
+// each wavefront walks 9 of the 27 nodes.
+template <int PL> struct Slots {
+  static constexpr int value = PL > 1 ? 64 : kBlock;  // particle slots per workgroup
+  static constexpr int per_chunk = kBlock / value;    // workgroups per work-list chunk
+};
+
 // Batched substeps record the node box each P2G workgroup scattered into
 // (8 ints per workgroup: x0, y0, z0, tx, ty, tz, 0, 0), so the G2P pass that
 // clears that grid for its next use zeroes only those boxes. A workgroup
@@ -460,7 +469,8 @@ struct NodeBox {
   }
 };
 
-__device__ NodeBox workgroup_box(bool act, const int base[3], int (*red)[kBlock / 64]) {
+template <int NW>
+__device__ NodeBox workgroup_box(bool act, const int base[3], int (*red)[NW]) {
   constexpr int kBig = 1 << 30;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int mn[3], mx[3];
@@ -482,7 +492,7 @@ __device__ NodeBox workgroup_box(bool act, const int base[3], int (*red)[kBlock 
     mn[a] = red[a][0];
     mx[a] = red[3 + a][0];
 #pragma unroll
-    for (int wv = 1; wv < kBlock / 64; ++wv) {
+    for (int wv = 1; wv < NW; ++wv) {
       mn[a] = min(mn[a], red[a][wv]);
       mx[a] = max(mx[a], red[3 + a][wv]);
     }
@@ -512,7 +522,7 @@ __device__ NodeBox workgroup_box(bool act, const int base[3], int (*red)[kBlock 
 // of fsi/coupling.py:332-406 fused in front when COUPLE). Lane particle p
 // (live: a particle of this workgroup). REGS: x, v and C come from the
 // caller's registers (the fused G2P+P2G launch) instead of memory.
-template <typename R, bool COUPLE, bool P2G, bool WIDE, bool REGS>
+template <typename R, bool COUPLE, bool P2G, bool WIDE, bool REGS, int PL>
 __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, int p, bool live, R* __restrict__ x,
                                          R* __restrict__ v, const R* __restrict__ C, R* __restrict__ Fd,
                                          const R* __restrict__ mass, const R* __restrict__ vol,
@@ -530,7 +540,14 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   // coupling: lattice velocity and a fluid flag per node of the box (zero
   // velocity and flag on solid and out-of-lattice nodes)
   __shared__ __align__(16) R ctile[COUPLE ? kN : 1][4];
-  __shared__ int red[6][kBlock / 64];
+  constexpr int kS = Slots<PL>::value;
+  __shared__ int red[6][PL * kS / 64];
+  // plane split (PL = 3): partial support and interpolated velocity
+  __shared__ R cpart[PL > 1 && COUPLE ? PL : 1][PL > 1 && COUPLE ? kS : 1][4];
+  static_assert(PL == 1 || (PL == 3 && sizeof(R) == 4), "plane split: FP32 only");
+  const int pln = PL > 1 ? (int)threadIdx.x / kS : 0;
+  const int slot = PL > 1 ? (int)threadIdx.x % kS : (int)threadIdx.x;
+  const bool writer = pln == 0;  // the wavefront that writes the particle
   const bool act = live && active[p] != 0;
   const int cap = m.cap;
   const int64_t N = (int64_t)m.nx * m.ny * m.nz;
@@ -576,7 +593,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     mn[a] = red[a][0];
     mx[a] = red[3 + a][0];
 #pragma unroll
-    for (int wv = 1; wv < kBlock / 64; ++wv) {
+    for (int wv = 1; wv < PL * kS / 64; ++wv) {
       mn[a] = min(mn[a], red[a][wv]);
       mx[a] = max(mx[a], red[3 + a][wv]);
     }
@@ -659,21 +676,43 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   rotate3<A>(ri, w[0][0], w[1][0], w[2][0], wir);
   rotate3<A>(rj, w[0][1], w[1][1], w[2][1], wjr);
   rotate3<A>(rk, w[0][2], w[1][2], w[2][2], wkr);
+  // plane split: this wavefront's x plane of the stencil is the first (and
+  // only) one of the static order
+  constexpr int kPlanes = PL > 1 ? 1 : 3;
+  if constexpr (PL > 1) {
+    io[0] = pln;
+    plane[0] = pln * tyz;
+    wir[0] = A(pick3(pln, w[0][0], w[1][0], w[2][0]));
+  }
 
   if constexpr (COUPLE) {
-    if (live && !act) {
+    if (live && !act && writer) {
       valid_weight[p] = R(0);
       cmask[p] = 0;
     }
-    if (act) {
-      // support of the valid (in-lattice, fluid) nodes and the interpolated
-      // lattice velocity, in i, j, k order; the nodes are gathered in batches
-      // of NB i-planes, all loads of a batch in flight together
-      R wsum = R(0), u[3] = {R(0), R(0), R(0)};
-      uint32_t valid = 0;  // global path: bit q = (i*3+j)*3+k
-      if (use_lds) {
-        // staged values are zero on invalid nodes and the flag is 0 or 1, so
-        // the products are the masked sums without selects
+    // support of the valid (in-lattice, fluid) nodes and the interpolated
+    // lattice velocity, in i, j, k order (with the plane split: this
+    // wavefront's x plane, the three planes' sums added in plane order)
+    R wsum = R(0), u[3] = {R(0), R(0), R(0)};
+    uint32_t valid = 0;  // global path: bit q = (i*3+j)*3+k
+    if (act && use_lds) {
+      // staged values are zero on invalid nodes and the flag is 0 or 1, so
+      // the products are the masked sums without selects
+      if constexpr (PL > 1) {
+        const R wi = pick3(pln, w[0][0], w[1][0], w[2][0]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const R* c = ctile[b0 + pln * tyz + j * tzi + k];
+            const R c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+            const R wt = wi * w[j][1] * w[k][2];
+            wsum += wt * c3;
+            u[0] += wt * c0;
+            u[1] += wt * c1;
+            u[2] += wt * c2;
+          }
+      } else {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -689,6 +728,23 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
               u[2] += wt * c2;
             }
       }
+    }
+    if constexpr (PL > 1) {
+      if (use_lds) {  // uniform over the workgroup
+        cpart[pln][slot][0] = wsum;
+        cpart[pln][slot][1] = u[0];
+        cpart[pln][slot][2] = u[1];
+        cpart[pln][slot][3] = u[2];
+        __syncthreads();
+        wsum = (cpart[0][slot][0] + cpart[1][slot][0]) + cpart[2][slot][0];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) u[a] = (cpart[0][slot][1 + a] + cpart[1][slot][1 + a]) + cpart[2][slot][1 + a];
+      }
+    }
+    if (act) {
+      // global path (box too large for the tile): the nodes are gathered in
+      // batches of NB i-planes, all loads of a batch in flight together;
+      // every plane of a split gathers all 27
       constexpr int NB = WIDE ? 3 : 1;  // i-planes per gather batch (global path)
 #pragma unroll 1
       for (int i0 = use_lds ? 3 : 0; i0 < 3; i0 += NB) {
@@ -729,7 +785,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
             }
         }
       }
-      valid_weight[p] = wsum;
+      if (writer) valid_weight[p] = wsum;
       if (wsum > cp.min_w) {
         partial = wsum < R(1) - R(1e-5);
         R dv[3];
@@ -760,7 +816,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
             clipped = true;
           }
         }
-        cmask[p] = 1;
+        if (writer) cmask[p] = 1;
         // equal and opposite reaction (and occupancy) spread on the valid nodes
         const R inv_wsum = R(1) / wsum;  // one division, 27 products below
         const R fscale = cp.dt_ratio / cp.cell_volume;
@@ -787,7 +843,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
               off[k] = (4 + c) * kN;
             }
 #pragma unroll
-            for (int ii = 0; ii < 3; ++ii)
+            for (int ii = 0; ii < kPlanes; ++ii)
 #pragma unroll
               for (int jj = 0; jj < 3; ++jj) {
                 const A wij = wir[ii] * wjr[jj];
@@ -805,7 +861,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
             spread(std::true_type{});
           else
             spread(std::false_type{});
-        } else {
+        } else if (writer) {
 #pragma unroll
           for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -824,13 +880,15 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
         }
       } else {
         unsupported = true;
-        cmask[p] = 0;
+        if (writer) cmask[p] = 0;
       }
     }
-    wave_count(counters + 0, unsupported);
-    wave_count(counters + 1, partial);
-    wave_count(counters + 2, clipped);
-    if (live) {
+    if (writer) {
+      wave_count(counters + 0, unsupported);
+      wave_count(counters + 1, partial);
+      wave_count(counters + 2, clipped);
+    }
+    if (live && writer) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) pforce[a * cap + p] = pf[a];
       if (cp.force_mean) {
@@ -840,7 +898,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     }
   } else {
     if (clear_force) {
-      if (live) {
+      if (live && writer) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) pforce[a * cap + p] = R(0);
       }
@@ -853,6 +911,9 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   // FP32 particles whose F needs the FP64 SVD are scattered after the flush,
   // where little else is live (keeps the kernel at four waves per SIMD)
   bool deferred = false;
+  // plane split: every plane reads F before plane 0 stores the update, so
+  // the store waits for the flush barrier
+  R Fn[9];
   if constexpr (P2G) {
     if (act) {
 #pragma unroll
@@ -860,10 +921,11 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
         Cm[r] = REGS ? rC[r] : C[r * cap + p];
         Fo[r] = Fd[r * cap + p];
       }
-      R Fn[9];
       deformation_update(m.dt, Cm, Fo, Fn);
+      if constexpr (PL == 1) {
 #pragma unroll
-      for (int r = 0; r < 9; ++r) Fd[r * cap + p] = Fn[r];
+        for (int r = 0; r < 9; ++r) Fd[r * cap + p] = Fn[r];
+      }
       R Rot[9];
       bool have_rot = true;
       if constexpr (sizeof(R) == 8)
@@ -900,7 +962,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
             off[k] = c * kN;
           }
 #pragma unroll
-          for (int ii = 0; ii < 3; ++ii) {
+          for (int ii = 0; ii < kPlanes; ++ii) {
             const A dpi = (A(io[ii]) - A(fx[0])) * dx;
             A mi[4];
 #pragma unroll
@@ -921,7 +983,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
               }
             }
           }
-        } else {
+        } else if (writer) {
           scatter_global(m, grid, base, fx, w, mom0, aff, mp);
         }
       }
@@ -953,11 +1015,18 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
       }
     }
   }
+  if constexpr (P2G && PL > 1) {
+    if (!use_lds) __syncthreads();  // (the tiled path synchronised at the flush)
+    if (act && writer) {
+#pragma unroll
+      for (int r = 0; r < 9; ++r) Fd[r * cap + p] = Fn[r];
+    }
+  }
   if constexpr (P2G && sizeof(R) == 4) {
     // inverted or ill-conditioned F: reload the particle (F is the updated
     // one stored above, pforce holds this substep's coupling force) and use
     // the FP64 Jacobi rotation; global atomics, as on the untiled path
-    if (deferred) {
+    if (deferred && writer) {
       R xr[3], vr[3], fr[3], wr[3][3], Cr[9], Fr[9], pr[3];
       int br[3];
 #pragma unroll
@@ -987,8 +1056,8 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   }
 }
 
-template <typename R, bool COUPLE, bool P2G, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
+template <typename R, bool COUPLE, bool P2G, bool WIDE, int PL>
+__global__ __launch_bounds__(PL * Slots<PL>::value) void k_p2g(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
                                                 R* __restrict__ v, const R* __restrict__ C,
                                                 R* __restrict__ Fd, const R* __restrict__ mass,
                                                 const R* __restrict__ vol,
@@ -1009,16 +1078,20 @@ __global__ __launch_bounds__(kBlock) void k_p2g(MpmK<R> m, CplK<R> cp, R* __rest
   // particle of this lane: from the work list (one chunk of at most 256
   // particles of two z-adjacent 4x4x4 cell blocks per workgroup), else slot
   // order
+  // PL wavefront groups share the particles of the workgroup (plane split)
+  constexpr int kS = Slots<PL>::value, kSub = Slots<PL>::per_chunk;
+  const int slot = (int)threadIdx.x % kS;
   int p;
   bool live;
   if (chunks) {
-    p = chunks[2 * blockIdx.x] + threadIdx.x;
-    live = (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
+    const int ch = blockIdx.x / kSub, sub = (blockIdx.x % kSub) * kS + slot;
+    p = chunks[2 * ch] + sub;
+    live = sub < chunks[2 * ch + 1];
   } else {
-    p = blockIdx.x * blockDim.x + threadIdx.x;
+    p = blockIdx.x * kS + slot;
     live = p < m.num;
   }
-  p2g_pass<R, COUPLE, P2G, WIDE, false>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid, solid, lbm_vel,
+  p2g_pass<R, COUPLE, P2G, WIDE, false, PL>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid, solid, lbm_vel,
                                         cforce, valid_weight, cmask, counters, clear_force, boxes_out, big_flag,
                                         big_value, closure, nullptr, nullptr, nullptr);
 }
@@ -1109,16 +1182,26 @@ __device__ __forceinline__ void zero_grid_pass(const MpmK<R>& m, uint4* __restri
 
 // Grid update + G2P pass of one workgroup (fsi/mpm3d.py:268-328, + the
 // contact pass of fsi/coupling.py:463-496 when CONTACT) for lane particle p.
-// Stores x, v, C and returns them in xo, vo, Co (zeros for lanes without an
-// active particle) for a P2G pass in the same launch.
-template <typename R, bool CONTACT, bool WIDE>
+// Stores x, v and C and returns them in xo, vo, Co (zeros for lanes without
+// an active particle) for a P2G pass in the same launch. PL = 3: the
+// workgroup has three wavefronts per particle slot; the one of plane
+// `plane` gathers the 9 nodes of x offset `plane`, the partial sums of v and
+// C are added in plane order through LDS, every plane continues with the
+// same totals and plane 0 alone writes the particle.
+template <typename R, bool CONTACT, bool WIDE, int PL>
 __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, int p, bool live, R* __restrict__ x,
                                          R* __restrict__ v, R* __restrict__ C, const int32_t* __restrict__ active,
                                          const R* __restrict__ grid, const int32_t* __restrict__ solid,
                                          const R* __restrict__ cforce, int32_t* __restrict__ contact_mask,
                                          int32_t* __restrict__ counters, R xo[3], R vo[3], R Co[9]) {
+  static_assert(PL == 1 || (PL == 3 && sizeof(R) == 4), "plane split: FP32 only");
   __shared__ R vtile[kG2PNodes][3];
-  __shared__ int red[6][kBlock / 64];
+  constexpr int kS = Slots<PL>::value;
+  __shared__ int red[6][PL * kS / 64];
+  __shared__ R gpart[PL > 1 ? PL : 1][PL > 1 ? kS : 1][12];
+  const int plane = PL > 1 ? (int)threadIdx.x / kS : 0;
+  const int slot = PL > 1 ? (int)threadIdx.x % kS : (int)threadIdx.x;
+  const bool writer = plane == 0;
   const bool act = live && active[p] != 0;
   const int cap = m.cap;
   bool cand = false, damped = false;
@@ -1134,7 +1217,7 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
   // stage the grid velocity of the workgroup's node box in LDS: the grid
   // update (fsi/mpm3d.py:268-288) runs once per node instead of once per
   // particle and node, and the 27-node gathers read LDS
-  const NodeBox box = workgroup_box(act, base, red);
+  const NodeBox box = workgroup_box<PL * kS / 64>(act, base, red);
   const bool staged = box.nodes > 0 && box.nodes <= kG2PNodes;
   if (staged) {
     for (int t = threadIdx.x; t < (int)box.nodes; t += blockDim.x) {
@@ -1153,13 +1236,13 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
     __syncthreads();
   }
 
+  R nv[3] = {R(0), R(0), R(0)};
+  R nc[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
   if (act) {
-    R nv[3] = {R(0), R(0), R(0)};
-    R nc[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
     // one node's contribution, in the reference's i, j, k order
-    auto accumulate = [&](int i, int j, int k, const R gv[3]) {
+    auto accumulate = [&](int i, R wi, int j, int k, const R gv[3]) {
       const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
-      const R wt = w[i][0] * w[j][1] * w[k][2];
+      const R wt = wi * w[j][1] * w[k][2];
 #pragma unroll
       for (int a = 0; a < 3; ++a) nv[a] += wt * gv[a];
       const R s = m.four_inv_dx * wt;
@@ -1169,22 +1252,35 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
         for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
     };
     if (staged) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
+      if constexpr (PL > 1) {
+        const R wi = pick3(plane, w[0][0], w[1][0], w[2][0]);
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            const R* tv = vtile[box.index(base[0] + i, base[1] + j, base[2] + k)];
+            const R* tv = vtile[box.index(base[0] + plane, base[1] + j, base[2] + k)];
             const R gv[3] = {tv[0], tv[1], tv[2]};
-            accumulate(i, j, k, gv);
+            accumulate(plane, wi, j, k, gv);
           }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const R* tv = vtile[box.index(base[0] + i, base[1] + j, base[2] + k)];
+              const R gv[3] = {tv[0], tv[1], tv[2]};
+              accumulate(i, w[i][0], j, k, gv);
+            }
+      }
     } else {
       // box too large for LDS (particles far apart): gather from the grid in
-      // batches of NB i-planes, all loads of a batch in flight together
-      constexpr int NB = WIDE ? 3 : 1;
+      // batches of NB i-planes, all loads of a batch in flight together (one
+      // plane per wavefront with the plane split)
+      constexpr int NB = PL > 1 ? 1 : (WIDE ? 3 : 1);
 #pragma unroll 1
-      for (int i0 = 0; i0 < 3; i0 += NB) {
+      for (int i0 = PL > 1 ? plane : 0; i0 < (PL > 1 ? plane + 1 : 3); i0 += NB) {
         R gn[NB * 9][4];
         uint32_t inmask = 0;
 #pragma unroll
@@ -1215,26 +1311,33 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
 #pragma unroll
               for (int a = 0; a < 3; ++a) gv[a] = in ? gv[a] : R(0);
               // w[i][0] with i dynamic when NB == 1
-              const R wi = pick3(i, w[0][0], w[1][0], w[2][0]);
-              const R dp[3] = {R(i) - fx[0], R(j) - fx[1], R(k) - fx[2]};
-              const R wt = wi * w[j][1] * w[k][2];
-#pragma unroll
-              for (int a = 0; a < 3; ++a) nv[a] += wt * gv[a];
-              const R s = m.four_inv_dx * wt;
-#pragma unroll
-              for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
+              accumulate(i, pick3(i, w[0][0], w[1][0], w[2][0]), j, k, gv);
             }
       }
     }
+  }
+  if constexpr (PL > 1) {
+    // the three planes' partial sums, added in plane order
 #pragma unroll
-    for (int r = 0; r < 9; ++r) C[r * cap + p] = m.apic ? nc[r] : R(0);
+    for (int a = 0; a < 3; ++a) gpart[plane][slot][a] = nv[a];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) gpart[plane][slot][3 + r] = nc[r];
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 3; ++a) nv[a] = (gpart[0][slot][a] + gpart[1][slot][a]) + gpart[2][slot][a];
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+      nc[r] = (gpart[0][slot][3 + r] + gpart[1][slot][3 + r]) + gpart[2][slot][3 + r];
+  }
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+      if (writer) C[r * cap + p] = m.apic ? nc[r] : R(0);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const R xn = xp[a] + m.dt * nv[a];
       xp[a] = fmin(fmax(xn, m.lower), m.upper[a]);
-      x[a * cap + p] = xp[a];
+      if (writer) x[a * cap + p] = xp[a];
     }
     if constexpr (CONTACT) {
       cand = contact_test(m, cp, xp, solid, cforce);
@@ -1245,7 +1348,8 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
       }
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) v[a * cap + p] = nv[a];
+    for (int a = 0; a < 3; ++a)
+      if (writer) v[a * cap + p] = nv[a];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       xo[a] = xp[a];
@@ -1260,14 +1364,16 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
     for (int r = 0; r < 9; ++r) Co[r] = R(0);
   }
   if constexpr (CONTACT) {
-    if (live) contact_mask[p] = cand ? 1 : 0;
-    wave_count(counters + 3, cand);
-    wave_count(counters + 4, damped);
+    if (writer) {
+      if (live) contact_mask[p] = cand ? 1 : 0;
+      wave_count(counters + 3, cand);
+      wave_count(counters + 4, damped);
+    }
   }
 }
 
-template <typename R, bool CONTACT, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
+template <typename R, bool CONTACT, bool WIDE, int PL>
+__global__ __launch_bounds__(PL * Slots<PL>::value) void k_g2p(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
                                                 R* __restrict__ v, R* __restrict__ C,
                                                 const int32_t* __restrict__ active,
                                                 const R* __restrict__ grid,
@@ -1286,18 +1392,22 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
   if (zero_ctr && gt < 3) zero_ctr[gt] = 0;
   // particle of this lane: the work list of the particle sort (workgroups
   // beyond it only zero), else slot order
+  constexpr int kS = Slots<PL>::value, kSub = Slots<PL>::per_chunk;
+  const int slot = (int)threadIdx.x % kS;
   int p;
   bool live;
   if (chunks) {
-    const bool has = (int)blockIdx.x < num_chunks;
-    p = has ? chunks[2 * blockIdx.x] + threadIdx.x : 0;
-    live = has && (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
+    const int ch = blockIdx.x / kSub, sub = (blockIdx.x % kSub) * kS + slot;
+    const bool has = ch < num_chunks;
+    p = has ? chunks[2 * ch] + sub : 0;
+    live = has && sub < chunks[2 * ch + 1];
   } else {
-    p = gt;
+    p = blockIdx.x * kS + slot;
     live = p < m.num;
   }
   R xo[3], vo[3], Co[9];
-  g2p_pass<R, CONTACT, WIDE>(m, cp, p, live, x, v, C, active, grid, solid, cforce, contact_mask, counters, xo, vo, Co);
+  g2p_pass<R, CONTACT, WIDE, PL>(m, cp, p, live, x, v, C, active, grid, solid, cforce, contact_mask, counters, xo, vo,
+                                 Co);
 }
 
 // Substeps s-1 and s back to back in one launch: the grid update + G2P of
@@ -1306,8 +1416,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p(MpmK<R> m, CplK<R> cp, R* __rest
 // grid_dst, zeroed before the launch). The work list is the same for both
 // passes; the zeroing part prepares the grid of substep s+1. Workgroups
 // beyond the particle blocks only zero.
-template <typename R, bool COUPLE, bool CONTACT, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_g2p_p2g(
+template <typename R, bool COUPLE, bool CONTACT, bool WIDE, int PL>
+__global__ __launch_bounds__(PL * Slots<PL>::value) void k_g2p_p2g(
     MpmK<R> m, CplK<R> cp, R* __restrict__ x, R* __restrict__ v, R* __restrict__ C, R* __restrict__ Fd,
     const R* __restrict__ mass, const R* __restrict__ vol, const int32_t* __restrict__ active,
     R* __restrict__ pforce, const R* __restrict__ grid_src, R* __restrict__ grid_dst,
@@ -1319,20 +1429,23 @@ __global__ __launch_bounds__(kBlock) void k_g2p_p2g(
     int32_t* __restrict__ boxes_out, int32_t* __restrict__ big_out, int big_value, const R* __restrict__ closure) {
   zero_grid_pass<R>(m, zero_grid, zero_n16, boxes_in, big_in, big_expect, num_boxes);
   const bool has = (int)blockIdx.x < particle_blocks;
+  constexpr int kS = Slots<PL>::value, kSub = Slots<PL>::per_chunk;
+  const int slot = (int)threadIdx.x % kS;
   int p;
   bool live;
   if (chunks) {
-    p = has ? chunks[2 * blockIdx.x] + threadIdx.x : 0;
-    live = has && (int)threadIdx.x < chunks[2 * blockIdx.x + 1];
+    const int ch = blockIdx.x / kSub, sub = (blockIdx.x % kSub) * kS + slot;
+    p = has ? chunks[2 * ch] + sub : 0;
+    live = has && sub < chunks[2 * ch + 1];
   } else {
-    p = blockIdx.x * blockDim.x + threadIdx.x;
+    p = blockIdx.x * kS + slot;
     live = p < m.num;
   }
   R xo[3], vo[3], Co[9];
-  g2p_pass<R, CONTACT, WIDE>(m, cp, p, live, x, v, C, active, grid_src, solid, cforce, contact_mask, g2p_counters,
-                             xo, vo, Co);
+  g2p_pass<R, CONTACT, WIDE, PL>(m, cp, p, live, x, v, C, active, grid_src, solid, cforce, contact_mask, g2p_counters,
+                                 xo, vo, Co);
   if (has)  // uniform over the workgroup
-    p2g_pass<R, COUPLE, true, WIDE, true>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid_dst, solid,
+    p2g_pass<R, COUPLE, true, WIDE, true, PL>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid_dst, solid,
                                           lbm_vel, cforce, valid_weight, cmask, p2g_counters, clear_force, boxes_out,
                                           big_out, big_value, closure, xo, vo, Co);
 }
@@ -1477,6 +1590,22 @@ int check_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
   return 0;
 }
 
+// Plane split of the particle passes (three wavefronts per particle, one per
+// x plane of the stencil): FP32, a work list of at most split_max_chunks
+// chunks (one-batch gathers).
+template <typename R> bool plane_split(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
+  return sizeof(R) == 4 && b->chunks && p->num_chunks > 0 && p->num_chunks <= p->split_max_chunks &&
+         (!b->zero_boxes || (int64_t)Slots<3>::per_chunk * p->num_chunks <= (int64_t)b->zero_boxes_cap);
+}
+
+// Particle workgroups of one P2G or G2P pass (one node box each).
+template <typename R> int64_t particle_workgroups(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
+  if (p->num_particles == 0) return 0;
+  if (b->chunks && p->num_chunks > 0)
+    return (int64_t)p->num_chunks * (plane_split<R>(p, b) ? Slots<3>::per_chunk : 1);
+  return blocks_for(p->num_particles);
+}
+
 template <typename R>
 int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
           const fsi_coupling_buffers* cb, int clear_force, int do_p2g, bool zero_counters,
@@ -1494,7 +1623,7 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
   if (m.num == 0) return 0;
   // with a work list from the particle sort, one workgroup per chunk
   const int32_t* chunks = (b->chunks && p->num_chunks > 0) ? b->chunks : nullptr;
-  const dim3 grid(chunks ? (unsigned)p->num_chunks : blocks_for(m.num)), block(kBlock);
+  const dim3 grid((unsigned)particle_workgroups<R>(p, b)), block(kBlock);
 #define FSI_P2G_ARGS                                                                            \
   m, cp, (R*)b->x, (R*)b->v, (const R*)b->C, (R*)b->F, (const R*)b->mass, (const R*)b->volume, \
       b->active, (R*)b->pforce, grid_target, cb ? cb->solid : nullptr,                          \
@@ -1506,12 +1635,19 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
   // gathers' registers) takes the one-batch gathers: its time is one
   // workgroup's latency
   const bool wide = grid.x <= (unsigned)kWideGridLimit;
-#define FSI_P2G_LAUNCH(C, P)                                                              \
-  do {                                                                                    \
-    if (wide)                                                                             \
-      hipLaunchKernelGGL((k_p2g<R, C, P, true>), grid, block, 0, st, FSI_P2G_ARGS);       \
-    else                                                                                  \
-      hipLaunchKernelGGL((k_p2g<R, C, P, false>), grid, block, 0, st, FSI_P2G_ARGS);      \
+  const bool split = plane_split<R>(p, b);
+#define FSI_P2G_LAUNCH(C, P)                                                                          \
+  do {                                                                                                \
+    if constexpr (sizeof(R) == 4) {                                                                   \
+      if (split) {                                                                                    \
+        hipLaunchKernelGGL((k_p2g<R, C, P, false, 3>), grid, dim3(3 * Slots<3>::value), 0, st, FSI_P2G_ARGS); \
+        break;                                                                                        \
+      }                                                                                               \
+    }                                                                                                 \
+    if (wide)                                                                                         \
+      hipLaunchKernelGGL((k_p2g<R, C, P, true, 1>), grid, block, 0, st, FSI_P2G_ARGS);                \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_p2g<R, C, P, false, 1>), grid, block, 0, st, FSI_P2G_ARGS);               \
   } while (0)
   if (couple && do_p2g)
     FSI_P2G_LAUNCH(true, true);
@@ -1548,7 +1684,7 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   // with a work list from the particle sort, one workgroup per chunk (as P2G)
   const int32_t* chunks = (b->chunks && p->num_chunks > 0) ? b->chunks : nullptr;
   const int num_chunks = chunks ? p->num_chunks : 0;
-  const int64_t particle_blocks = m.num ? (chunks ? (int64_t)num_chunks : (int64_t)blocks_for(m.num)) : 0;
+  const int64_t particle_blocks = particle_workgroups<R>(p, b);
   const int64_t blocks = std::max<int64_t>(particle_blocks, std::max<int64_t>(zero_blocks, zero_ctr ? 1 : 0));
   if (blocks == 0) return 0;
   // particle workgroups that fit on the chip at once take the one-batch
@@ -1560,14 +1696,27 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
       CONTACT_ON ? ctrs : nullptr, (uint4*)zero_grid, n16, zero_ctr, chunks, num_chunks,            \
       boxes_in, big_flag, big_expect, num_boxes
   const dim3 g2p_grid((unsigned)blocks), g2p_block(kBlock);
-  if (contact && wide)
-    hipLaunchKernelGGL((k_g2p<R, true, true>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
+  bool launched = false;
+  if constexpr (sizeof(R) == 4) {
+    if (plane_split<R>(p, b)) {
+      if (contact)
+        hipLaunchKernelGGL((k_g2p<R, true, false, 3>), g2p_grid, dim3(3 * Slots<3>::value), 0, st,
+                           FSI_G2P_ARGS(true));
+      else
+        hipLaunchKernelGGL((k_g2p<R, false, false, 3>), g2p_grid, dim3(3 * Slots<3>::value), 0, st,
+                           FSI_G2P_ARGS(false));
+      launched = true;
+    }
+  }
+  if (launched) {
+  } else if (contact && wide)
+    hipLaunchKernelGGL((k_g2p<R, true, true, 1>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
   else if (contact)
-    hipLaunchKernelGGL((k_g2p<R, true, false>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
+    hipLaunchKernelGGL((k_g2p<R, true, false, 1>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
   else if (wide)
-    hipLaunchKernelGGL((k_g2p<R, false, true>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(false));
+    hipLaunchKernelGGL((k_g2p<R, false, true, 1>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(false));
   else
-    hipLaunchKernelGGL((k_g2p<R, false, false>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(false));
+    hipLaunchKernelGGL((k_g2p<R, false, false, 1>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(false));
 #undef FSI_G2P_ARGS
   FSI_TRY_LAUNCH("mpm_g2p");
   return 0;
@@ -1635,6 +1784,7 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
   const int64_t zero_blocks = std::min<int64_t>(blocks_for(n16), 1024);
   const dim3 grid((unsigned)std::max<int64_t>(num_wg, zero_blocks)), block(kBlock);
   const bool wide = num_wg <= kWideGridLimit;
+  const bool split = plane_split<R>(p, b);
   for (int s = 1; s < substeps; ++s) {
     const int q = (s + 1) % 3;
     const bool by_boxes = s >= 2;
@@ -1645,12 +1795,19 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
       cb ? cb->coupling_mask : nullptr, cb ? cb->contact_mask : nullptr, row(s - 1), row(s), clear_force,      \
       chunks, (int)num_wg, (uint4*)G[q], n16, by_boxes ? boxes_of(q) : nullptr, by_boxes ? flag_of(q) : nullptr, \
       s - 1, (int)num_wg, boxes_of(s % 3), flag_of(s % 3), s + 1, cb ? (const R*)cb->closure_force : nullptr
-#define FSI_FUSED_LAUNCH(CPL, CON)                                                                      \
-  do {                                                                                                  \
-    if (wide)                                                                                           \
-      hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, true>), grid, block, 0, st, FSI_FUSED_ARGS);           \
-    else                                                                                                \
-      hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false>), grid, block, 0, st, FSI_FUSED_ARGS);          \
+#define FSI_FUSED_LAUNCH(CPL, CON)                                                                        \
+  do {                                                                                                    \
+    if constexpr (sizeof(R) == 4) {                                                                       \
+      if (split) {                                                                                        \
+        hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false, 3>), grid, dim3(3 * Slots<3>::value), 0, st,     \
+                           FSI_FUSED_ARGS);                                                               \
+        break;                                                                                            \
+      }                                                                                                   \
+    }                                                                                                     \
+    if (wide)                                                                                             \
+      hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, true, 1>), grid, block, 0, st, FSI_FUSED_ARGS);          \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false, 1>), grid, block, 0, st, FSI_FUSED_ARGS);         \
   } while (0)
     if (couple && contact)
       FSI_FUSED_LAUNCH(true, true);
@@ -1693,9 +1850,7 @@ int substeps_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   // dirty boxes: P2G pass s records its workgroups' boxes under parity s%2;
   // G2P pass s >= 1 zeroes the other grid from the boxes of pass s-1 (pass 0
   // zeroes it whole: its contents are unknown)
-  const int64_t num_wg = p->num_particles == 0 ? 0
-                         : (b->chunks && p->num_chunks > 0) ? (int64_t)p->num_chunks
-                                                            : (int64_t)blocks_for(p->num_particles);
+  const int64_t num_wg = particle_workgroups<R>(p, b);
   const bool boxes = two && b->zero_boxes && num_wg <= (int64_t)b->zero_boxes_cap;
   // three grids, box scratch for three passes and counter rows: one launch
   // per substep boundary

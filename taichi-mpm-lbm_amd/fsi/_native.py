@@ -59,7 +59,7 @@ class MpmParams(ctypes.Structure):
         ("boundary_damping", ctypes.c_double),
         ("apic", ctypes.c_int32),
         ("clamp_lower", ctypes.c_double), ("clamp_upper", c_double3),
-        ("num_chunks", ctypes.c_int32),
+        ("num_chunks", ctypes.c_int32), ("split_max_chunks", ctypes.c_int32),
     ]
 
 

@@ -82,6 +82,10 @@ class MPMSolver3D:
         self._sort = None
         self._permuted = False
         self._num_chunks = 0  # P2G work list length after a sort (0: slot order)
+        # FP32 substeps with a work list of at most this many chunks run three
+        # wavefronts per particle (one per x plane of the stencil): a small
+        # particle set is bound by one wavefront's latency, not by throughput
+        self.split_max_chunks = 256
         self.num_particles = 0
         self._buffers = self._make_buffers()
         self._initialized = False
@@ -229,6 +233,7 @@ class MPMSolver3D:
         p.boundary_damping = self.boundary_damping
         p.apic = 1 if self.apic else 0
         p.num_chunks = self._num_chunks if self._permuted else 0
+        p.split_max_chunks = self.split_max_chunks
         p.clamp_lower = 0.5 * self.dx
         p.clamp_upper = nat.c_double3((self.nx - 1.5) * self.dx, (self.ny - 1.5) * self.dx,
                                       (self.nz - 1.5) * self.dx)

@@ -370,3 +370,34 @@ def test_coupled_substep_paths_without_third_grid(path, coupled_oracle):
     for k in ("unsupported_particle_count", "partial_support_particle_count", "clipped_particle_count",
               "ib_active_cell_count", "contact_candidate_count", "contact_damped_particle_count"):
         assert got["diag"][k] == ref["diag"][k], (k, got["diag"][k], ref["diag"][k])
+
+
+def test_plane_split_matches_the_one_wavefront_passes():
+    """FP32 substeps of a small particle set run three wavefronts per
+    particle (one per x plane of the stencil, partial sums added in plane
+    order); with the split off the same scene runs one wavefront per
+    particle. Only the order of the FP32 sums over the stencil differs:
+    fields agree to FP32 rounding growth over the run, counters exactly or
+    by one particle at a threshold."""
+    import fsi
+
+    cfg, solid, pos, vel = coupled_scene(storage="fp32", ib=True, contact=True)
+    runs = []
+    for split in (256, 0):
+        sim = fsi.FSISimulation(cfg)
+        sim.initialize_lbm(solid)
+        sim.initialize_mpm_from_numpy(pos, vel, particle_mass=1.2 * 0.125, particle_volume=0.125)
+        sim.mpm.split_max_chunks = split
+        for _ in range(COUPLED["penalty_ib_contact"]["steps"]):
+            sim.step()
+        f = sim_fields(sim)
+        f["diag"] = sim.coupler.coupling_diagnostics()
+        runs.append(f)
+    a, b = runs
+    for key in ("x", "v", "F", "pforce", "u", "rho", "cforce", "phi"):
+        err = rel_inf(a[key], b[key])
+        print(f"split vs one-wavefront {key}: rel {err:.3e}")
+        assert err <= 2e-5, (key, err)
+    for k in ("unsupported_particle_count", "partial_support_particle_count", "clipped_particle_count",
+              "contact_candidate_count", "contact_damped_particle_count"):
+        assert abs(a["diag"][k] - b["diag"][k]) <= 1, (k, a["diag"][k], b["diag"][k])
