@@ -231,3 +231,27 @@ def test_step_force_conserves_momentum_with_the_lattice():
     sim.step()
     with pytest.raises(RuntimeError):
         sim.coupler.step_force_numpy()
+
+
+def test_half_resolution_run_tracks_the_refined_run():
+    """The plain coarse run of a held-out ellipsoid scene (48x24x24) against
+    its 2x-refined run: mean drag (the fluid's substep-averaged force on the
+    solid) and centroid displacement within 3%, the accuracy the
+    half-resolution runs are held to; the closure in the loop keeps them
+    there; outside the training envelope are the first step's evaluations
+    (the flow starts at the inflow speed around bodies at rest) and at most
+    a few percent of the rest."""
+    from fsi.closure import ClosureScene, accuracy_vs_refined, scene_pair_data, train_closure
+
+    scene = ClosureScene.random(400, shape=(48, 24, 24), axes=(3.0, 5.0))
+    data = [scene_pair_data(ClosureScene.random(s, shape=(48, 24, 24), axes=(3.0, 5.0)), 30, every=2, warmup=6)
+            for s in (300, 301, 302)]
+    closure, _ = train_closure(np.concatenate([d[0] for d in data]), np.concatenate([d[1] for d in data]),
+                               steps=400, batch=8192, lr=2e-3, seed=1)
+    acc = accuracy_vs_refined(scene, 30, closure)
+    print(acc)
+    for name in ("plain", "closure"):
+        assert acc[name]["drag_rel_err"] <= 0.03, (name, acc[name])
+        assert acc[name]["displacement_rel_err"] <= 0.03, (name, acc[name])
+    per_step = acc["closure"]["evaluations"] // 30
+    assert acc["closure"]["fallbacks"] <= per_step + 0.03 * (acc["closure"]["evaluations"] - per_step)
