@@ -145,10 +145,10 @@ typedef struct fsi_mpm_params {
                               particles in slot order, 256 per workgroup) */
   int32_t split_max_chunks; /* FP32 with a work list of at most this many chunks:
                               each particle gets three wavefronts, one per x plane
-                              of its 27-node stencil, in the P2G/G2P passes of
-                              fsi_mpm_substeps (small particle sets, whose passes
-                              are bound by one wavefront's instruction latency);
-                              0 disables */
+                              of its 27-node stencil, in the P2G and G2P passes
+                              (small particle sets, whose passes are bound by one
+                              wavefront's instruction latency); results differ
+                              only in the order of FP32 sums; 0 disables */
 } fsi_mpm_params;
 
 typedef struct fsi_mpm_buffers {
