@@ -255,3 +255,19 @@ def test_half_resolution_run_tracks_the_refined_run():
         assert acc[name]["displacement_rel_err"] <= 0.03, (name, acc[name])
     per_step = acc["closure"]["evaluations"] // 30
     assert acc["closure"]["fallbacks"] <= per_step + 0.03 * (acc["closure"]["evaluations"] - per_step)
+
+
+def test_step_force_of_a_non_stepping_evaluation():
+    """compute_coupling_forces (one coupling pass, no step) leaves the
+    accumulator at that pass's force times dt_ratio: exactly the particle
+    force times 0.25, whatever it held before."""
+    from fsi.closure import ClosureScene
+
+    sim = ClosureScene.random(14, shape=(24, 12, 12), axes=(2.5, 3.5)).simulation(False)
+    sim.coupler.record_step_force()
+    sim.step()
+    sim.coupler.compute_coupling_forces(dt_ratio=0.25)
+    fm = sim.coupler.step_force_numpy()
+    pf = sim.mpm.particle_forces_numpy()
+    assert np.abs(pf).max() > 0
+    np.testing.assert_array_equal(fm, pf * np.float32(0.25))
