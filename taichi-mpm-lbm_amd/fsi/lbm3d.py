@@ -113,6 +113,16 @@ class LBMSolver3D:
 
     def _validate_supported_config(self) -> None:
         # fsi/lbm3d.py:85-108
+        c = self.config
+        pop_bytes = {"fp32": 4, "fp16": 2, "fp64": 8}.get(getattr(c, "storage", "fp32"), 4)
+        if _Q * int(c.nx) * int(c.ny) * int(c.nz) * pop_bytes >= 2 ** 32:
+            # the lattice kernels address a population buffer with 32-bit
+            # byte offsets (one VGPR per address, SGPR base)
+            raise NotImplementedError(
+                f"a {c.nx}x{c.ny}x{c.nz} lattice needs {_Q * pop_bytes} B x cells >= 4 GiB per population "
+                f"buffer in {getattr(c, 'storage', 'fp32')} storage; the lattice kernels support up to "
+                f"{(2 ** 32 - 1) // (_Q * pop_bytes):,} cells (use fp16 storage or a smaller lattice)."
+            )
         if self.config.use_sparse:
             raise NotImplementedError("Sparse LBM storage is not implemented in Step 2.")
         for name, boundary in (("x_left", self.config.x_left), ("x_right", self.config.x_right)):

@@ -87,6 +87,12 @@ def test_unsupported_lattice_features_raise_before_device_work():
         LBMSolver3D(LBMConfig(**{**periodic, "y_left": BoundaryConfig("pressure")}))
     with pytest.raises(NotImplementedError):
         LBMSolver3D(LBMConfig(**{**periodic, "x_left": BoundaryConfig("wall")}))
+    # 32-bit byte offsets into a population buffer: 384^3 FP32 is one cell
+    # past the limit, FP16 halves the bytes (raised before any allocation)
+    with pytest.raises(NotImplementedError, match="4 GiB"):
+        LBMSolver3D(LBMConfig(nx=384, ny=384, nz=384, **periodic))
+    with pytest.raises(NotImplementedError, match="4 GiB"):
+        LBMSolver3D(LBMConfig(nx=512, ny=512, nz=512, storage="fp16", **periodic))
 
 
 def test_solvers_fail_loudly_without_gpu():
