@@ -167,8 +167,9 @@ def coupled_scene_32(storage="fp32"):
 
 
 def coupled_scene_large(storage="fp32"):
-    """256x128x128 coupled suspension: inflow u_x=0.04 (velocity face x-),
-    pressure face x+, no-slip walls as a solid shell in y/z, tau=0.56;
+    """256x128x128 coupled suspension: inflow u_x=0.04 (velocity face x-,
+    fluid starting at that velocity), pressure face x+, no-slip walls as a
+    solid shell in y/z, tau=0.56;
     16 non-overlapping elastic spheres r U[6,12] (sphere_pack seed 0),
     8 particles per cell (spacing 0.5, jitter U(-0.125,0.125)), E=1e3,
     nu=0.3, density 1.2, gamma=0.5, 112 substeps per lattice step."""
@@ -177,7 +178,11 @@ def coupled_scene_large(storage="fp32"):
 
     shape = (256, 128, 128)
     pos, info = sphere_pack(shape, 16, 6.0, 12.0, spacing=0.5, seed=0)
+    # the fluid starts at the inflow velocity: started from rest, the
+    # reference's velocity face on this channel becomes unstable after ~140
+    # steps (reproduced by the CPU oracle in the reference's operation order)
     lbm = LBMConfig(nx=shape[0], ny=shape[1], nz=shape[2], viscosity=0.06 / 3.0, rho0=1.0,
+                    initial_velocity=(0.04, 0.0, 0.0),
                     x_left=BoundaryConfig("velocity", velocity=(0.04, 0.0, 0.0)),
                     x_right=BoundaryConfig("pressure", rho=1.0), storage=storage)
     mpm = MPMConfig(nx=shape[0], ny=shape[1], nz=shape[2], max_particles=len(pos), dx=1.0, dt=1.0 / 112,
@@ -188,7 +193,10 @@ def coupled_scene_large(storage="fp32"):
     return cfg, channel_walls(shape), pos
 
 
-def time_coupled_large(steps, warmup, storage="fp32"):
+def time_coupled_large(steps, warmup, storage="fp32", fields=None):
+    """Times the suspension; with `fields` (a dict) also stores the lattice
+    velocity and the particle positions and velocities after the run."""
+    import numpy as np
     import torch
 
     import fsi
@@ -205,6 +213,13 @@ def time_coupled_large(steps, warmup, storage="fp32"):
         sim.advance()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    if fields is not None:
+        # untimed: continue to fields["steps"] coupled steps, then record
+        for _ in range(int(fields.get("steps", 0)) - warmup - steps):
+            sim.advance()
+        fields["u"] = sim.lbm.velocity_numpy().astype(np.float64)
+        fields["x"] = sim.mpm.positions_numpy().astype(np.float64)
+        fields["v"] = sim.mpm.velocities_numpy().astype(np.float64)
     n_sub = cfg.coupling.mpm_substeps_per_lbm_step
     return {"steps_per_s": round(steps / dt, 3), "ms_per_step": round(1e3 * dt / steps, 3),
             "particles": int(len(pos)), "substeps": n_sub, "timed_steps": steps,
@@ -428,13 +443,25 @@ def main():
         line["coupled"] = {"workload": "32^3 coupled penalty scene, FP32, 4096 particles, 112 substeps",
                            **time_coupled(args.coupled_steps, 2)}
     if args.large_steps > 0:
+        f32, f16 = {"steps": 200}, {"steps": 200}
         line["coupled_large"] = {
             "workload": "256x128x128 coupled suspension, FP32, 16 elastic spheres, 112 substeps",
-            **time_coupled_large(args.large_steps, 1)}
+            **time_coupled_large(args.large_steps, 1, fields=f32)}
         # mixed precision: FP16 shifted populations, FP32 MPM and coupling
         line["coupled_large_fp16"] = {
             "workload": "256x128x128 coupled suspension, FP16 lattice storage, FP32 MPM, 112 substeps",
-            **time_coupled_large(args.large_steps, 1, storage="fp16")}
+            **time_coupled_large(args.large_steps, 1, storage="fp16", fields=f16)}
+        # the mixed-precision run against the FP32 run after the same steps
+        # (max-norm relative error per field; particle positions relative to
+        # the lattice extent, displacements relative to the largest one)
+        disp = np.abs(f32["x"] - np.asarray(coupled_scene_large()[2], np.float64)).max()
+        line["coupled_large_fp16"]["error_vs_fp32"] = {
+            "steps": f32["steps"],
+            "u_rel_max": float(np.abs(f16["u"] - f32["u"]).max() / max(np.abs(f32["u"]).max(), 1e-30)),
+            "particle_v_rel_max": float(np.abs(f16["v"] - f32["v"]).max() / max(np.abs(f32["v"]).max(), 1e-30)),
+            "particle_x_abs_max_cells": float(np.abs(f16["x"] - f32["x"]).max()),
+            "particle_x_rel_to_max_displacement": float(np.abs(f16["x"] - f32["x"]).max() / max(disp, 1e-30)),
+        }
     if not args.no_closure:
         line["learned_closure"] = time_closure()
     if not args.no_cpu_baseline:
