@@ -100,15 +100,6 @@ def test_rigid_translation_keeps_F_identity_and_C_zero():
     assert np.allclose(s.v[:n], [0.1, -0.05, 0.02], atol=1e-12)
 
 
-def test_quadratic_weights_partition_of_unity():
-    """The weights used by p2g/g2p sum to 1 and have zero first moment about fx."""
-    rng = np.random.default_rng(0)
-    for fx in rng.uniform(0.5, 1.5, 20):
-        w = np.array([0.5 * (1.5 - fx) ** 2, 0.75 - (fx - 1) ** 2, 0.5 * (fx - 0.5) ** 2])
-        assert w.sum() == pytest.approx(1.0, abs=1e-15)
-        assert (w * (np.arange(3) - fx)).sum() == pytest.approx(0.0, abs=1e-15)
-
-
 def test_polar_rotation_properties():
     rng = np.random.default_rng(3)
     for i in range(200):
