@@ -149,6 +149,13 @@ typedef struct fsi_mpm_params {
                               (small particle sets, whose passes are bound by one
                               wavefront's instruction latency); results differ
                               only in the order of FP32 sums; 0 disables */
+  int32_t wide_max_workgroups; /* particle passes of at most this many workgroups gather
+                              all 27 stencil nodes in one batch (one memory round trip,
+                              more registers: a launch that fits on the chip at once);
+                              larger ones gather one x plane of 9 nodes at a time and the
+                              FP32 fused G2P+P2G kernel is held to 128 VGPRs. Results are
+                              the same up to the order of float atomics; 0: always the
+                              plane-batch kernels */
 } fsi_mpm_params;
 
 typedef struct fsi_mpm_buffers {

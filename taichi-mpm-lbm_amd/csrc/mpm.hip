@@ -405,10 +405,6 @@ __global__ __launch_bounds__(kBlock) void k_clear(int64_t n, R* __restrict__ buf
   if (counters && t < 5) counters[t] = 0;
 }
 
-// Grids of at most this many workgroups run the gathers in one batch (see
-// k_p2g): 2 resident workgroups per CU x 256 CUs.
-constexpr int kWideGridLimit = 512;
-
 // Plane split (PL = 3): a workgroup takes 64 particle slots (a quarter of a
 // work-list chunk) with three wavefronts, one per x plane of the stencil, so
 // the passes of a small particle set spread over four times as many CUs and
@@ -1471,6 +1467,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && PL == 1 && !WIDE ? 4 : 1)))
   R xo[3], vo[3], Co[9];
   g2p_pass<R, CONTACT, WIDE, PL, pass_lds_bytes<R, COUPLE>()>(m, cp, p, live, x, v, C, active, grid_src, solid, cforce,
                                                           contact_mask, g2p_counters, xo, vo, Co);
+  // the P2G stage reuses the G2P stage's LDS buffer: every read of the
+  // staged grid velocities completes before any thread writes its tile
+  __syncthreads();
   if (has)  // uniform over the workgroup
     p2g_pass<R, COUPLE, true, WIDE, true, PL>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid_dst, solid,
                                           lbm_vel, cforce, valid_weight, cmask, p2g_counters, clear_force, boxes_out,
@@ -1661,7 +1660,7 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
   // a grid that fits on the chip at once (2 workgroups per CU with the wide
   // gathers' registers) takes the one-batch gathers: its time is one
   // workgroup's latency
-  const bool wide = grid.x <= (unsigned)kWideGridLimit;
+  const bool wide = (int64_t)grid.x <= (int64_t)p->wide_max_workgroups;
   const bool split = plane_split<R>(p, b);
 #define FSI_P2G_LAUNCH(C, P)                                                                          \
   do {                                                                                                \
@@ -1716,7 +1715,7 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   if (blocks == 0) return 0;
   // particle workgroups that fit on the chip at once take the one-batch
   // gathers (as for P2G)
-  const bool wide = particle_blocks <= kWideGridLimit;
+  const bool wide = particle_blocks <= (int64_t)p->wide_max_workgroups;
 #define FSI_G2P_ARGS(CONTACT_ON)                                                                   \
   m, cp, (R*)b->x, (R*)b->v, (R*)b->C, b->active, grid_src, CONTACT_ON ? cb->solid : nullptr,      \
       CONTACT_ON ? (const R*)cb->cforce : nullptr, CONTACT_ON ? cb->contact_mask : nullptr,        \
@@ -1810,7 +1809,7 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
   const int64_t n16 = (4LL * p->nx * p->ny * p->nz * (int64_t)sizeof(R)) / 16;
   const int64_t zero_blocks = std::min<int64_t>(blocks_for(n16), 1024);
   const dim3 grid((unsigned)std::max<int64_t>(num_wg, zero_blocks)), block(kBlock);
-  const bool wide = num_wg <= kWideGridLimit;
+  const bool wide = num_wg <= (int64_t)p->wide_max_workgroups;
   const bool split = plane_split<R>(p, b);
   for (int s = 1; s < substeps; ++s) {
     const int q = (s + 1) % 3;

@@ -60,6 +60,7 @@ class MpmParams(ctypes.Structure):
         ("apic", ctypes.c_int32),
         ("clamp_lower", ctypes.c_double), ("clamp_upper", c_double3),
         ("num_chunks", ctypes.c_int32), ("split_max_chunks", ctypes.c_int32),
+        ("wide_max_workgroups", ctypes.c_int32),
     ]
 
 

@@ -88,6 +88,10 @@ class MPMSolver3D:
         # (measured on MI355X: 0.66x the time at 33-51 chunks, 0.83x at 112,
         # 1.23x at 168)
         self.split_max_chunks = 128
+        # particle passes of at most this many workgroups (2 per CU on the
+        # 256 CUs) gather the 27 stencil nodes in one batch; larger ones a
+        # plane of 9 at a time with fewer registers (0: always the latter)
+        self.wide_max_workgroups = 512
         self.num_particles = 0
         self._buffers = self._make_buffers()
         self._initialized = False
@@ -236,6 +240,7 @@ class MPMSolver3D:
         p.apic = 1 if self.apic else 0
         p.num_chunks = self._num_chunks if self._permuted else 0
         p.split_max_chunks = self.split_max_chunks
+        p.wide_max_workgroups = self.wide_max_workgroups
         p.clamp_lower = 0.5 * self.dx
         p.clamp_upper = nat.c_double3((self.nx - 1.5) * self.dx, (self.ny - 1.5) * self.dx,
                                       (self.nz - 1.5) * self.dx)
