@@ -259,6 +259,22 @@ int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
                      const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
                      int32_t substeps, int32_t build_first, int32_t clear_force, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). Same as
+ * fsi_mpm_substeps, but b->zero_boxes / b->zero_boxes_cap and
+ * cb->counter_rows / cb->counter_rows_cap are ignored: the node boxes and
+ * counter rows live in a static device array of the library (6.4 MB, one
+ * per GPU, held until the process ends). Limits: at most 65536 particle
+ * workgroups per pass (the sort's chunk count, four per chunk with the plane
+ * split, or ceil(num_particles / 256) unsorted) and, when coupling or
+ * contact counters are kept, at most 4096 substeps per call; larger calls
+ * fail with FSI_ERR_ARGUMENT. Calls no allocating routine. Every call on one
+ * GPU shares the array: calls that overlap on one GPU (other streams or
+ * threads) race. */
+int fsi_mpm_substeps_noalloc(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
+                             const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
+                             int32_t substeps, int32_t build_first, int32_t clear_force, void* stream);
+
 /* Device workspace (bytes) fsi_mpm_sort needs for this grid and capacity. */
 int fsi_mpm_sort_workspace(const fsi_mpm_params* p, int64_t* bytes);
 

@@ -7,6 +7,8 @@
 //
 // Per substep: clear grid -> P2G (+ coupling) -> grid update + G2P (+ contact).
 
+#include <mutex>
+
 #include "common.h"
 
 namespace fsi {
@@ -1922,6 +1924,74 @@ int contact_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupl
   return 0;
 }
 
+// ------------------------------------------- allocation-free substeps ----
+//
+// fsi_mpm_substeps_noalloc keeps the substep loop's scratch (the node boxes of
+// three P2G passes and the per-substep counter rows) in one file-scope device
+// array instead of the caller's fsi_mpm_buffers.zero_boxes and
+// fsi_coupling_buffers.counter_rows. Every GPU holds its own copy until the
+// process ends; calls on one GPU share it, so calls that overlap on one GPU
+// race. Its stale contents are harmless: each call writes the boxes before
+// they are read and zeroes the counter rows with a kernel, and a box-size
+// flag left over from an earlier call can only select a whole-grid clear.
+
+constexpr int kNoAllocMaxWorkgroups = 65536;  // P2G workgroups of one pass
+constexpr int kNoAllocMaxSubsteps = 4096;     // substeps of one call with counters
+constexpr int64_t kNoAllocBoxInts = 16 + 24LL * kNoAllocMaxWorkgroups;
+constexpr int64_t kNoAllocScratchInts = kNoAllocBoxInts + 8LL * kNoAllocMaxSubsteps;  // 6.4 MB
+
+}  // namespace
+}  // namespace fsi
+
+static __device__ __align__(256) int32_t scratch[fsi::kNoAllocScratchInts];
+
+namespace fsi {
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device (host code must not pass the symbol itself to a kernel: its host-side
+// name is not a device address).
+int noalloc_scratch(int32_t** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static int32_t* ptrs[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("mpm_substeps_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    return FSI_ERR_HIP;
+  }
+  FSI_REQUIRE(dev >= 0 && dev < kMaxDevices, "mpm_substeps_noalloc: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("mpm_substeps_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return FSI_ERR_HIP;
+    }
+    ptrs[dev] = static_cast<int32_t*>(p);
+  }
+  *out = ptrs[dev];
+  return 0;
+}
+
+int check_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+                   const fsi_coupling_buffers* cb, int32_t substeps, const char* name) {
+  if (int rc = check_mpm(p, b)) return rc;
+  FSI_REQUIRE(substeps >= 0, "%s: negative substep count", name);
+  FSI_REQUIRE(b->x && b->v && b->C && b->F && b->mass && b->volume && b->active && b->pforce && b->grid,
+              "%s: null particle or grid buffer", name);
+  if (c && c->enabled)
+    FSI_REQUIRE(cb && cb->solid && cb->lbm_vel && cb->cforce && cb->valid_weight && cb->coupling_mask &&
+                    cb->counters,
+                "%s: coupling needs all coupling buffers", name);
+  if (c && c->contact_enabled)
+    FSI_REQUIRE(cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters,
+                "%s: contact needs the coupling buffers", name);
+  return 0;
+}
+
 }  // namespace
 }  // namespace fsi
 
@@ -1980,26 +2050,44 @@ int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_cou
 int fsi_mpm_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
                      const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
                      int32_t substeps, int32_t build_first, int32_t clear_force, void* stream) {
-  if (int rc = check_mpm(p, b)) return rc;
-  FSI_REQUIRE(substeps >= 0, "mpm_substeps: negative substep count");
-  FSI_REQUIRE(b->x && b->v && b->C && b->F && b->mass && b->volume && b->active && b->pforce &&
-                  b->grid,
-              "mpm_substeps: null particle or grid buffer");
-  const bool counters = c && (c->enabled || c->contact_enabled);
-  if (c && c->enabled)
-    FSI_REQUIRE(cb && cb->solid && cb->lbm_vel && cb->cforce && cb->valid_weight &&
-                    cb->coupling_mask && cb->counters,
-                "mpm_substeps: coupling needs all coupling buffers");
-  if (c && c->contact_enabled)
-    FSI_REQUIRE(cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters,
-                "mpm_substeps: contact needs the coupling buffers");
+  if (int rc = check_substeps(p, b, c, cb, substeps, "mpm_substeps")) return rc;
   hipStream_t st = (hipStream_t)stream;
-  fsi_coupling_params local{};
-  if (c) local = *c;
-  const bool f64 = p->real == FSI_REAL_F64;
+  const bool counters = c && (c->enabled || c->contact_enabled);
   int32_t* ctr = counters ? cb->counters : nullptr;
-  if (f64) return substeps_t<double>(p, b, c, cb, ctr, substeps, build_first, clear_force, st);
+  if (p->real == FSI_REAL_F64) return substeps_t<double>(p, b, c, cb, ctr, substeps, build_first, clear_force, st);
   return substeps_t<float>(p, b, c, cb, ctr, substeps, build_first, clear_force, st);
+}
+
+int fsi_mpm_substeps_noalloc(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
+                             const fsi_coupling_buffers* cb, int32_t substeps, int32_t build_first,
+                             int32_t clear_force, void* stream) {
+  if (int rc = check_substeps(p, b, c, cb, substeps, "mpm_substeps_noalloc")) return rc;
+  const int64_t wg = p->real == FSI_REAL_F64 ? particle_workgroups<double>(p, b) : particle_workgroups<float>(p, b);
+  FSI_REQUIRE(wg <= kNoAllocMaxWorkgroups,
+              "mpm_substeps_noalloc: %lld particle workgroups per pass exceed the static scratch's %d "
+              "(use fsi_mpm_substeps with caller-provided zero_boxes)",
+              (long long)wg, kNoAllocMaxWorkgroups);
+  const bool counters = c && (c->enabled || c->contact_enabled);
+  FSI_REQUIRE(!counters || substeps <= kNoAllocMaxSubsteps,
+              "mpm_substeps_noalloc: %d substeps exceed the static counter rows' %d", substeps,
+              kNoAllocMaxSubsteps);
+  int32_t* s = nullptr;
+  if (int rc = noalloc_scratch(&s)) return rc;
+  fsi_mpm_buffers lb = *b;
+  lb.zero_boxes = s;
+  lb.zero_boxes_cap = kNoAllocMaxWorkgroups;
+  fsi_coupling_buffers lcb{};
+  if (cb) {
+    lcb = *cb;
+    lcb.counter_rows = s + kNoAllocBoxInts;
+    lcb.counter_rows_cap = kNoAllocMaxSubsteps;
+  }
+  const fsi_coupling_buffers* pcb = cb ? &lcb : nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  int32_t* ctr = counters ? cb->counters : nullptr;
+  if (p->real == FSI_REAL_F64)
+    return substeps_t<double>(p, &lb, c, pcb, ctr, substeps, build_first, clear_force, st);
+  return substeps_t<float>(p, &lb, c, pcb, ctr, substeps, build_first, clear_force, st);
 }
 
 int fsi_closure_features(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_buffers* cb,

@@ -111,6 +111,7 @@ _SIGNATURES = {
     "fsi_mpm_p2g": [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp],
     "fsi_mpm_grid_g2p": [_vp, _vp, _vp, _vp, _vp],
     "fsi_mpm_substeps": [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp],
+    "fsi_mpm_substeps_noalloc": [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp],
     "fsi_contact": [_vp, _vp, _vp, _vp, _vp],
     "fsi_polar_rotation": [ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp],
     "fsi_force_assemble": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp],
