@@ -39,10 +39,8 @@ __all__ = [
     "extract_snapshot_timeseries",
     "list_npz_snapshots",
     "load_reference_dataset",
-    "load_validation_summary",
     "run_reference_validation_suite",
     "run_validation_suite",
-    "validation_summary_table",
 ]
 
 # name -> (module, attribute); the solver modules load torch and the kernel
@@ -56,8 +54,6 @@ _LAZY = {
     "SnapshotInfo": ("postprocess", "SnapshotInfo"),
     "extract_snapshot_timeseries": ("postprocess", "extract_snapshot_timeseries"),
     "list_npz_snapshots": ("postprocess", "list_npz_snapshots"),
-    "load_validation_summary": ("postprocess", "load_validation_summary"),
-    "validation_summary_table": ("postprocess", "validation_summary_table"),
     "ReferenceDataset": ("reference", "ReferenceDataset"),
     "ReferenceReport": ("reference", "ReferenceReport"),
     "load_reference_dataset": ("reference", "load_reference_dataset"),

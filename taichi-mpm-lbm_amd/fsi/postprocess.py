@@ -1,5 +1,6 @@
-"""Readers for the NPZ snapshots and validation summaries (reference:
-fsi/postprocess.py). Host-side numpy only.
+"""Readers for the NPZ snapshots (reference: fsi/postprocess.py:32-115).
+Host-side numpy only; the reference's CSV/JSON writers, validation tables and
+plotting are offline tooling around the step and are not part of this package.
 
 Snapshots are `snapshot_<step>.npz` files written by fsi.output. The time
 series derived from them use the reference's keys and definitions
@@ -10,8 +11,6 @@ position, so tables and plots built on either implementation agree.
 
 from __future__ import annotations
 
-import csv
-import json
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Any
@@ -36,9 +35,6 @@ _NORMS = (("coupling_force", "coupling_force_norm"), ("immersed_boundary_force",
           ("total_particle_coupling_force", "total_particle_coupling_force_norm"),
           ("total_fluid_coupling_force", "total_fluid_coupling_force_norm"),
           ("ib_total_force", "ib_total_force_norm"))
-_SUMMARY_FIELDS = ("case_name", "case_passed", "metric_name", "value", "lower", "upper", "metric_passed",
-                   "units", "description")
-
 
 @dataclass(frozen=True)
 class SnapshotInfo:
@@ -60,18 +56,6 @@ def _scalar(v, kind):
 def _step_of(path: Path):
     tail = path.stem[len("snapshot_"):] if path.stem.startswith("snapshot_") else ""
     return (0, int(tail)) if tail.isdigit() else (1, path.name)
-
-
-def _plain(v: Any) -> Any:
-    if isinstance(v, dict):
-        return {k: _plain(x) for k, x in v.items()}
-    if isinstance(v, (list, tuple)):
-        return [_plain(x) for x in v]
-    if isinstance(v, np.ndarray):
-        return v.tolist()
-    if isinstance(v, np.generic):
-        return v.item()
-    return v
 
 
 def list_npz_snapshots(output_dir: str | Path) -> list[Path]:
@@ -148,80 +132,3 @@ def summarize_coupling_diagnostics(output_dir: str | Path) -> dict[str, np.ndarr
             "total_fluid_coupling_force_norm", *_COUNTS[:3], "ib_force_norm", "ib_total_force_norm",
             *_COUNTS[3:])
     return {k: ts[k] for k in keys if k in ts}
-
-
-def _checked_keys(ts: dict[str, np.ndarray]) -> list[str]:
-    if not ts:
-        raise ValueError("time series must not be empty.")
-    keys = _ordered(ts)
-    if len({len(np.asarray(ts[k])) for k in keys}) != 1:
-        raise ValueError("all time series arrays must have the same length.")
-    return keys
-
-
-def write_timeseries_csv(timeseries: dict[str, np.ndarray], path: str | Path) -> Path:
-    keys = _checked_keys(timeseries)
-    out = Path(path)
-    out.parent.mkdir(parents=True, exist_ok=True)
-    with out.open("w", newline="", encoding="utf-8") as fh:
-        w = csv.DictWriter(fh, fieldnames=keys)
-        w.writeheader()
-        for i in range(len(np.asarray(timeseries[keys[0]]))):
-            w.writerow({k: _plain(np.asarray(timeseries[k])[i]) for k in keys})
-    return out
-
-
-def write_timeseries_json(timeseries: dict[str, np.ndarray], path: str | Path) -> Path:
-    _checked_keys(timeseries)
-    out = Path(path)
-    out.parent.mkdir(parents=True, exist_ok=True)
-    out.write_text(json.dumps(_plain(timeseries), indent=2), encoding="utf-8")
-    return out
-
-
-def load_validation_summary(path: str | Path) -> list[dict[str, Any]]:
-    data = json.loads(Path(path).read_text(encoding="utf-8"))
-    if not isinstance(data, list):
-        raise ValueError("validation summary JSON must contain a list of reports.")
-    return data
-
-
-def validation_summary_table(reports: list[dict[str, Any]]) -> list[dict[str, Any]]:
-    """One row per metric of each report dict (ValidationReport.to_dict())."""
-    return [{"case_name": r.get("case_name", ""), "case_passed": bool(r.get("passed", False)),
-             "metric_name": m.get("name", ""), "value": m.get("value"), "lower": m.get("lower"),
-             "upper": m.get("upper"), "metric_passed": bool(m.get("passed", False)),
-             "units": m.get("units", ""), "description": m.get("description", "")}
-            for r in reports for m in r.get("metrics", [])]
-
-
-def write_validation_summary_csv(rows: list[dict[str, Any]], path: str | Path) -> Path:
-    out = Path(path)
-    out.parent.mkdir(parents=True, exist_ok=True)
-    with out.open("w", newline="", encoding="utf-8") as fh:
-        w = csv.DictWriter(fh, fieldnames=list(_SUMMARY_FIELDS))
-        w.writeheader()
-        for r in rows:
-            w.writerow({k: _plain(r.get(k)) for k in _SUMMARY_FIELDS})
-    return out
-
-
-def plot_timeseries(timeseries: dict[str, np.ndarray], y_key: str, output_path: str | Path, *,
-                    x_key: str = "step") -> Path:
-    """PNG line plot of one key (needs matplotlib)."""
-    for k, role in ((x_key, "x_key"), (y_key, "y_key")):
-        if k not in timeseries:
-            raise KeyError(f"Missing {role} in time series: {k}.")
-    import matplotlib.pyplot as plt
-
-    out = Path(output_path)
-    out.parent.mkdir(parents=True, exist_ok=True)
-    fig, ax = plt.subplots()
-    ax.plot(np.asarray(timeseries[x_key]), np.asarray(timeseries[y_key]), marker="o")
-    ax.set_xlabel(x_key)
-    ax.set_ylabel(y_key)
-    ax.grid(True)
-    fig.tight_layout()
-    fig.savefig(out)
-    plt.close(fig)
-    return out

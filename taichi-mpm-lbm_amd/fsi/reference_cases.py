@@ -35,9 +35,9 @@ CREATED_BY = "fsi MI355X reference generator"
 
 
 def default_reference_data_dir() -> Path:
-    """The committed reference datasets of this repository (the reference's
-    data/reference/*.json, kept with the test fixtures)."""
-    return Path(__file__).resolve().parents[2] / "tests" / "golden" / "reference"
+    """The reference's committed datasets (its data/reference/*.json), shipped
+    as package data of fsi."""
+    return Path(__file__).resolve().parent / "data" / "reference"
 
 
 def _selected(report: ValidationReport, names: tuple[str, ...]) -> dict[str, float]:
