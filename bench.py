@@ -44,17 +44,47 @@ def algorithmic_bytes(n_fluid, n_cells, pop_bytes):
     return n_fluid * 2 * 19 * pop_bytes + n_cells * 1
 
 
+def warm_up(step, count, min_seconds=0.25):
+    """`count` untimed steps, continued until `min_seconds` have passed (the
+    clocks and caches reach their steady state in a fraction of a second; a
+    short count alone under-reports the fast storages)."""
+    import torch
+
+    t0 = time.perf_counter()
+    done = 0
+    while done < count or time.perf_counter() - t0 < min_seconds:
+        step()
+        done += 1
+        if done % 20 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    return done
+
+
+def timed_loop(step, steps):
+    """Wall time of exactly `steps` steps between two synchronisations, and
+    the same interval from HIP events on the launch stream."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a.record(stream)
+    for _ in range(steps):
+        step()
+    b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, a.elapsed_time(b) * 1e-3
+
+
 def time_lattice(solver, steps, warmup, launch_samples):
     import torch
 
-    for _ in range(warmup):
-        solver.step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        solver.step()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    warm_up(solver.step, warmup)
+    wall, _ = timed_loop(solver.step, steps)
     # per-launch kernel duration: event pairs on the stream the kernel runs on
     stream = torch.cuda.current_stream()
     pairs = []
@@ -71,11 +101,12 @@ def time_lattice(solver, steps, warmup, launch_samples):
 
 
 def lattice_errors_vs_fp64(n, mask, fields, steps):
-    """Per-field error of the timed FP32 / FP16 lattice runs against the FP64
-    build of the same kernels after the same number of steps (the FP64 build
-    is pinned to the CPU oracle at 1e-10): max-norm relative error of u, its
-    L2-relative error, and the max-norm error of rho (absolute and relative
-    to max |rho - rho0| of the FP64 run), computed on the device."""
+    """Per-field error of the timed lattice runs (continued, untimed, to
+    `steps` steps) against the FP64 build of the same kernels after the same
+    number of steps (the FP64 build is pinned to the CPU oracle at 1e-10):
+    max-norm relative error of u, its L2-relative error, and the max-norm
+    error of rho (absolute and relative to max |rho - rho0| of the FP64 run),
+    computed on the device."""
     import torch
 
     import fsi
@@ -88,13 +119,15 @@ def lattice_errors_vs_fp64(n, mask, fields, steps):
     rho64, vel64 = s.rho, s.vel
     drho64 = (rho64 - 1.0).abs().max().item()
     out = {"steps": steps, "reference": "FP64 build of the same kernels, same mask and steps"}
-    for storage, (rho, vel) in fields.items():
+    out["u_max"] = float(vel64.abs().max().item())
+    for storage, (rho, vel, extra) in fields.items():
         dv = vel.double() - vel64
         out[storage] = {
             "u_rel_max": float(dv.abs().max().item() / vel64.abs().max().item()),
             "u_rel_l2": float(dv.norm().item() / vel64.norm().item()),
             "rho_abs_max": float((rho.double() - rho64).abs().max().item()),
             "rho_rel_to_deviation": float((rho.double() - rho64).abs().max().item() / max(drho64, 1e-300)),
+            **extra,
         }
     del s
     torch.cuda.empty_cache()
@@ -112,36 +145,112 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(mask, budget_s=12.0, max_steps=40):
-    """The CPU oracle (C++ restatement in the reference's operation order,
-    FP32, OpenMP over the host cores) on the same 256^3 scene; bounded sample."""
+def _cpu_task(task, budget_s):
+    """One CPU measurement of the oracle (C++ restatement of the reference's
+    algorithm in its operation order, FP32), run in a child process so its
+    OpenMP thread count is its own. Prints one JSON object."""
     import numpy as np
 
+    from fsi.scenes import porous_spheres
     from oracle import pyoracle as po
 
     po.build()
+    out = {"threads": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))}
+    if task == "lattice":
+        mask, _ = porous_spheres((256, 256, 256), porosity=0.8, rmin=4.0, rmax=10.0, seed=0)
+        s = po.OracleLBM(periodic_lbm_config(256, "fp32"), "f32")
+        s.initialize(mask)
+        s.step()  # warm-up (page faults, thread start)
+        done, t0 = 0, time.perf_counter()
+        while done < 40 and (done == 0 or time.perf_counter() - t0 < budget_s):
+            s.step()
+            done += 1
+        dt = time.perf_counter() - t0
+        out.update(value=mask.size * done / dt / 1e6, unit="MLUPS", steps=done, seconds=dt)
+    elif task == "coupled_32":
+        cfg, solid, pos = coupled_scene_32()
+        sim = po.OracleSimulation(cfg, "f32")
+        sim.initialize_lbm(solid)
+        sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+        sim.step()
+        done, t0 = 0, time.perf_counter()
+        while done < 20 and (done == 0 or time.perf_counter() - t0 < budget_s):
+            sim.step()
+            done += 1
+        dt = time.perf_counter() - t0
+        out.update(value=done / dt, unit="coupled steps/s", steps=done, seconds=dt)
+    elif task == "coupled_large":
+        # the suspension's coupled step: one lattice step plus 112 MPM
+        # substeps with the coupling; timed as one lattice step and two
+        # coupled substeps, extrapolated to 112 substeps
+        import dataclasses
+
+        cfg, solid, pos = coupled_scene_large()
+        sub = cfg.coupling.mpm_substeps_per_lbm_step
+        cfg2 = dataclasses.replace(cfg, coupling=dataclasses.replace(cfg.coupling, mpm_substeps_per_lbm_step=2))
+        sim = po.OracleSimulation(cfg2, "f32")
+        sim.initialize_lbm(solid)
+        sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+        t0 = time.perf_counter()
+        sim.lbm.step()
+        t_lat = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sim.coupler.step(cfg.lbm_dt)  # 2 substeps + force assembly + a lattice step
+        t_two = time.perf_counter() - t0 - t_lat
+        per_sub = t_two / 2.0
+        per_step = t_lat + sub * per_sub
+        out.update(value=1.0 / per_step, unit="coupled steps/s", lattice_step_s=t_lat,
+                   substep_s=per_sub, extrapolated_step_s=per_step, particles=int(len(pos)),
+                   sample=f"1 lattice step + 1 coupled step of 2 substeps, extrapolated to {sub} substeps")
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(budget_s=10.0):
+    """The CPU oracle on the host cores beside the GPU run, in child processes:
+    the 256^3 lattice on all cores (the headline comparison) and on one core,
+    the 32^3 coupled scene on all cores and on one core, and the 256x128x128
+    suspension's coupled step on all cores (extrapolated from a sample)."""
+    import subprocess
+
     cores = len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    n = mask.shape[0]
-    s = po.OracleLBM(periodic_lbm_config(n, "fp32"), "f32")
-    s.initialize(mask)
-    s.step()  # warm-up (page faults, thread start)
-    done = 0
-    t0 = time.perf_counter()
-    while done < max_steps:
-        s.step()
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return {
-        "value": round(mask.size * done / dt / 1e6, 3),
+
+    def run(task, threads):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-task", task,
+                            "--cpu-budget", str(budget_s)], env=env, capture_output=True, text=True,
+                           timeout=600)
+        if r.returncode != 0:
+            return {"error": r.stderr[-400:]}
+        return json.loads(r.stdout.strip().splitlines()[-1])
+
+    lat = run("lattice", cores)
+    lat1 = run("lattice", 1)
+    c32 = run("coupled_32", cores)
+    c32_1 = run("coupled_32", 1)
+    large = run("coupled_large", cores)
+    main = {
+        "value": round(lat.get("value", float("nan")), 3),
         "unit": "MLUPS",
-        "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+        "cores": int(lat.get("threads", cores)),
         "kind": "port",
-        "sample": f"{done} timed steps (+1 warm-up) of the 256^3 porous lattice in the CPU oracle, "
-                  f"FP32, reference operation order; {dt:.1f} s",
+        "sample": f"{lat.get('steps')} timed steps (+1 warm-up) of the 256^3 porous lattice in the CPU oracle "
+                  f"(C++ restatement in the reference's operation order), FP32, OpenMP; "
+                  f"{lat.get('seconds', 0):.1f} s",
     }
+    more = {
+        "lattice_1_core": {"value": round(lat1.get("value", float("nan")), 3), "unit": "MLUPS", "cores": 1,
+                           "sample": f"{lat1.get('steps')} timed steps of the same lattice"},
+        "coupled_32": {"value": round(c32.get("value", float("nan")), 4), "unit": "coupled steps/s",
+                       "cores": int(c32.get("threads", cores)),
+                       "sample": f"{c32.get('steps')} coupled steps of the 32^3 scene (4096 particles, 112 substeps)"},
+        "coupled_32_1_core": {"value": round(c32_1.get("value", float("nan")), 4), "unit": "coupled steps/s",
+                              "cores": 1, "sample": f"{c32_1.get('steps')} coupled steps"},
+        "coupled_large": {"value": round(large.get("value", float("nan")), 5), "unit": "coupled steps/s",
+                          "cores": int(large.get("threads", cores)),
+                          "sample": large.get("sample"), "extrapolated_step_s": large.get("extrapolated_step_s"),
+                          "particles": large.get("particles")},
+    }
+    return main, more
 
 
 def coupled_scene_32(storage="fp32"):
@@ -340,13 +449,20 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--no-error-check", action="store_true",
-                    help="skip the accuracy runs: the FP64 lattice the FP32/FP16 lattice fields are compared "
-                         "with, and the 200-step FP32/mixed-precision suspension comparison")
+                    help="skip the accuracy runs: the FP64 lattice the FP32/FP16/I16 lattice fields are compared "
+                         "with, and the FP32/mixed-precision suspension comparison")
+    ap.add_argument("--error-steps", type=int, default=1000,
+                    help="steps of the (untimed) lattice accuracy runs against the FP64 build")
+    ap.add_argument("--cpu-task", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help=argparse.SUPPRESS)
     ap.add_argument("--large-steps", type=int, default=3,
                     help="timed steps of the 256x128x128 coupled suspension (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of one population buffer (counter calibration)")
     args = ap.parse_args()
+    if args.cpu_task:  # child process of cpu_baseline(): CPU only
+        _cpu_task(args.cpu_task, args.cpu_budget)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU sharding "
               "of the lattice is a design note, not part of this build.", file=sys.stderr)
@@ -370,7 +486,7 @@ def main():
     runs = {}
     fields = {}
     line_errors = None
-    storages = ["fp32"] if args.no_fp16 else ["fp32", "fp16"]
+    storages = ["fp32"] if args.no_fp16 else ["fp32", "fp16", "i16"]
     for storage in storages:
         s = fsi.LBMSolver3D(periodic_lbm_config(n, storage))
         s.initialize(mask)
@@ -385,13 +501,21 @@ def main():
             "bytes": alg,
             "achieved_gbs": alg / (launch_ms * 1e-3) / 1e9,
         }
-        if not args.no_error_check:
-            s._ensure_macro()
-            fields[storage] = (s.rho.clone(), s.vel.clone())
         del s
         torch.cuda.empty_cache()
+        if not args.no_error_check:
+            # untimed accuracy run of exactly --error-steps steps from initialize()
+            s = fsi.LBMSolver3D(periodic_lbm_config(n, storage))
+            s.initialize(mask)
+            for i in range(args.error_steps):
+                s.step()
+            s._ensure_macro()
+            extra = {"storage_range_used": round(s.storage_range(), 4)} if storage == "i16" else {}
+            fields[storage] = (s.rho.clone(), s.vel.clone(), extra)
+            del s
+            torch.cuda.empty_cache()
     if fields:
-        line_errors = lattice_errors_vs_fp64(n, mask, fields, args.warmup + args.steps + args.launch_samples)
+        line_errors = lattice_errors_vs_fp64(n, mask, fields, args.error_steps)
 
     r32 = runs["fp32"]
     traffic = read_traffic(f"lbm_step_{n}_fp32_pmc.txt")
@@ -427,16 +551,20 @@ def main():
         "mflups": round(r32["mflups"], 2),
         "launch_ms": round(r32["launch_ms"], 5),
     }
-    if "fp16" in runs:
-        r16 = runs["fp16"]
-        line["fp16"] = {
+    for storage, label in (("fp16", "IEEE binary16 of (f - w rho0) 2^15"),
+                           ("i16", "16-bit integers of (f - w rho0)/w with an adaptive scale")):
+        if storage not in runs:
+            continue
+        r16 = runs[storage]
+        line[storage] = {
+            "storage": label,
             "mlups": round(r16["mlups"], 2),
             "ms_per_step": round(r16["ms_per_step"], 5),
             "launch_ms": round(r16["launch_ms"], 5),
             "roofline_frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
             "achieved_gbs": round(r16["achieved_gbs"], 1),
             "speedup_vs_fp32": round(r16["mlups"] / r32["mlups"], 3),
-            "traffic": read_traffic(f"lbm_step_{n}_fp16_pmc.txt"),
+            "traffic": read_traffic(f"lbm_step_{n}_{storage}_pmc.txt"),
         }
     if line_errors is not None:
         line["error_vs_fp64"] = line_errors
@@ -467,7 +595,7 @@ def main():
     if not args.no_closure:
         line["learned_closure"] = time_closure()
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(mask)
+        line["cpu_baseline"], line["cpu_baseline_more"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 

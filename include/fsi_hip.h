@@ -37,8 +37,12 @@ enum {
 
 enum { FSI_REAL_F32 = 0, FSI_REAL_F64 = 1 };
 
-/* population storage: absolute FP32 (reference), shifted+scaled FP16, FP64 */
-enum { FSI_POP_F32 = 0, FSI_POP_F16 = 1, FSI_POP_F64 = 2 };
+/* population storage: absolute FP32 (reference), shifted+scaled FP16, FP64,
+ * shifted signed-normalised 16-bit integers with an adaptive scale (I16) */
+enum { FSI_POP_F32 = 0, FSI_POP_F16 = 1, FSI_POP_F64 = 2, FSI_POP_I16 = 3 };
+
+/* floats of the I16 storage's control block (fsi_lbm_buffers.store_ctrl) */
+#define FSI_STORE_CTRL_FLOATS 8320
 
 /* x-face boundary modes, fsi/lbm3d.py:16-27 */
 enum { FSI_BC_PERIODIC = 0, FSI_BC_PRESSURE = 1, FSI_BC_VELOCITY = 2 };
@@ -54,7 +58,16 @@ typedef struct fsi_lbm_params {
   int32_t left_mode, right_mode;
   double bc_rho[2];        /* x_left.rho, x_right.rho */
   double bc_vel[2][3];     /* x_left.velocity, x_right.velocity */
-  double pop_scale;        /* FP16 storage scale (power of two); ignored otherwise */
+  double pop_scale;        /* FP16 storage scale (power of two); I16: the scale of the
+                              populations fsi_lbm_init writes; ignored otherwise */
+  int32_t store_epoch;     /* I16: population writes so far (fsi_lbm_init writes epoch 0,
+                              each fsi_lbm_step call the next one); ignored otherwise */
+  int32_t store_pass;      /* I16, fsi_lbm_step: 0 a normal step (its scale adapted from the
+                              previous write's largest value); 1 a measuring pass that
+                              computes the step without writing anything but the largest
+                              value of what it would store (its slot of store_ctrl zeroed
+                              before); 2 the step after a measuring pass, written at a scale
+                              from that measurement. Ignored otherwise */
 } fsi_lbm_params;
 
 typedef struct fsi_lbm_buffers {
@@ -65,6 +78,12 @@ typedef struct fsi_lbm_buffers {
   void* force;             /* 3N real: per-cell body force (fsi/lbm3d.py:126) */
   void* face_vel;          /* 2 slots x 2 faces x 3 x ny*nz real: macroscopic velocity of the
                               x=0 / x=nx-1 cells, read by the next pressure boundary */
+  float* store_ctrl;       /* I16 storage: FSI_STORE_CTRL_FLOATS floats, zeroed before
+                              fsi_lbm_init: per write epoch (three rotating slots) the scale
+                              the populations were written with and the largest |value| as a
+                              fraction of the 16-bit range, and a fourth slot for a measuring
+                              pass (floats [6240, 8320)); each step adapts the scale from the
+                              previous write's maximum. Unused otherwise (may be null) */
 } fsi_lbm_buffers;
 
 /* Force seen by one step. When `field` is null, every fluid cell uses
@@ -117,9 +136,11 @@ int fsi_lbm_force_import(const fsi_lbm_params* p, const fsi_lbm_buffers* b, cons
                          void* stream);
 
 /* fsi/lbm3d.py:444-445 (distribution_numpy): populations as the reference's
- * absolute values, AoS (N,19), real. */
-int fsi_lbm_export_populations(const fsi_lbm_params* p, const void* pop, void* out_aos,
-                               void* stream);
+ * absolute values, AoS (N,19), real. b: the lattice buffers (I16 storage reads
+ * the scale of the current write epoch from b->store_ctrl; may be null for the
+ * other storages). */
+int fsi_lbm_export_populations(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop,
+                               void* out_aos, void* stream);
 
 /* SoA (ncomp, n) -> AoS (n, ncomp) transposition for the *_numpy accessors. */
 int fsi_soa_to_aos(int32_t real, int64_t n, int32_t ncomp, const void* soa, void* aos,

@@ -20,7 +20,7 @@ CSRC = HERE / "csrc"
 OUT_DIR = HERE / "fsi" / "_lib"
 LIB = OUT_DIR / "libfsi_hip.so"
 SOURCES = ["runtime.hip", "lbm.hip", "mpm.hip", "coupling.hip", "reduce.hip", "sort.hip", "closure.hip"]
-HEADERS = ["common.h", "mrt_tables.h"]
+HEADERS = ["common.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -71,32 +71,72 @@ template <typename R> __host__ __device__ constexpr R wq(int q) {
 //
 // Populations are kept as deviations d_q = F_q - b_q from a per-direction base
 // b_q. FP32/FP64 storage keeps the absolute value (b_q = 0) exactly as the
-// reference does; FP16 storage keeps (F_q - w_q rho0) * scale with a power of
-// two scale so that slow-flow deviations stay in the normal FP16 range
-// (shifted FP16 storage, Lehmann et al., Phys. Rev. E 106, 015308 (2022)).
+// reference does. The 16-bit storages keep the shift b_q = w_q rho0
+// (Lehmann et al., Phys. Rev. E 106, 015308 (2022)):
+//  - FP16: (F_q - w_q rho0) * scale in IEEE binary16, a fixed power-of-two
+//    scale keeping slow-flow deviations in the normal range (11-bit
+//    significand relative to each value);
+//  - I16: (F_q - w_q rho0) / w_q * s as signed normalised 16-bit integers
+//    (round(x * 32767), |x| <= 1), with a run-time scale s that the lattice
+//    kernels adapt to the largest stored value, so every population is
+//    resolved to 2^-15 of the field's largest normalised deviation.
+// load/store take the factor of the population's weight class (rest, axis,
+// diagonal), ld = 1/(store factor).
 
 template <typename S> struct PopStore;
 
 template <> struct PopStore<float> {
   using real = float;
   static constexpr bool shifted = false;
+  static constexpr bool adaptive = false;
   __device__ static float load(float s, float) { return s; }
-  __device__ static float store(float d, float) { return d; }
+  __device__ static float scaled(float d, float) { return d; }
+  __device__ static float store(float x) { return x; }
 };
 
 template <> struct PopStore<double> {
   using real = double;
   static constexpr bool shifted = false;
+  static constexpr bool adaptive = false;
   __device__ static double load(double s, double) { return s; }
-  __device__ static double store(double d, double) { return d; }
+  __device__ static double scaled(double d, double) { return d; }
+  __device__ static double store(double x) { return x; }
 };
 
 template <> struct PopStore<__half> {
   using real = float;
   static constexpr bool shifted = true;
-  // inv_scale / scale are exact powers of two
-  __device__ static float load(__half s, float inv_scale) { return __half2float(s) * inv_scale; }
-  __device__ static __half store(float d, float scale) { return __float2half_rn(d * scale); }
+  static constexpr bool adaptive = false;
+  // the factors are exact powers of two
+  __device__ static float load(__half s, float ld) { return __half2float(s) * ld; }
+  __device__ static float scaled(float d, float st) { return d * st; }
+  __device__ static __half store(float x) { return __float2half_rn(x); }
 };
+
+template <> struct PopStore<int16_t> {
+  using real = float;
+  static constexpr bool shifted = true;
+  static constexpr bool adaptive = true;
+  // ld = w_q / (32767 s); st = s / w_q
+  __device__ static float load(int16_t s, float ld) { return (float)s * ld; }
+  __device__ static float scaled(float d, float st) { return d * st; }
+  // v_cvt_pknorm_i16_f32: round(clamp(x, -1, 1) * 32767) to nearest
+  __device__ static int16_t store(float x) { return __builtin_amdgcn_cvt_pknorm_i16(x, 0.0f)[0]; }
+};
+
+// weight class of a D3Q19 direction: 0 rest, 1 axis, 2 diagonal
+__host__ __device__ constexpr int wclass(int q) { return q == 0 ? 0 : (q < 7 ? 1 : 2); }
+
+// Adaptive I16 storage: a control block of three slots (rotating with the
+// write epoch of the populations), each the scale the slot's populations were
+// written with and the largest |normalised value| of that write, as
+// kCtrlParts partial maxima on separate 128-byte lines (float bits, combined
+// with atomicMax on their unsigned bits), and a fourth slot for the maxima of
+// a measuring pass (a step run without stores, before the first writes).
+constexpr int kCtrlParts = 64;
+constexpr int kCtrlLine = 32;                              // floats per 128-byte line
+constexpr int kCtrlSlot = kCtrlLine * (kCtrlParts + 1);    // floats per slot
+constexpr int kCtrlMeasure = 3;                            // slot of the measuring pass
+constexpr int kCtrlFloats = 4 * kCtrlSlot;
 
 }  // namespace fsi

@@ -22,41 +22,45 @@
 // populations): exec-mask divergence around memory instructions was measured
 // to halve the achieved bandwidth on porous masks.
 //
-// Collision: f* = F + M^-1 r with r_k = -S_k (m_k - meq_k) + (1 - S_k/2) G_k,
-// algebraically the reference's m = M F; m -= S (m - meq); m += (1 - S/2) G;
-// f = M^-1 m (fsi/lbm3d.py:314-333). The moments use the even/odd structure of
-// M under e -> -e (pair sums and differences of opposite populations), the
-// Guo moments G_k their closed form in (u, F), and M^-1 = M^T (M M^T)^-1 with
-// the block-diagonal exact Gram inverse; conserved moments (S_k = 0) keep
-// r_k = G_k.
+// Collision (fsi/lbm3d.py:314-333: m = M F; m -= S (m - meq); m += (1 - S/2) G;
+// f = M^-1 m) in population space. Every even row of the reference's M
+// except the density relaxes at s_v and every odd row except the momentum at
+// s_o, and M maps even (odd) population vectors to even (odd) moments, so the
+// MRT update is exactly a two-relaxation-time one:
+//   f* = F - s_v (F - g)^+ - s_o (F - g)^- + w (e.F)
+//          + c_v w ((e.u)(e.F) - u.F) - (1 - c_v) (2/3) w u.F
+// with ^+ / ^- the halves (h_q +- h_opp(q))/2, c_v = 1 - s_v/2, and g the
+// populations whose moments are the reference's equilibrium moments with F's
+// own density and momentum: g = w (rho_F + 3 e.j + 4.5 (e.u)^2 - 1.5 u.u),
+// rho_F = sum F, j = sum e F (M^-1 applied to meq; the forcing terms are
+// M^-1 (1 - S/2) G with G = M s, s = w ((e - u).F + (e.u)(e.F))). A CPU test
+// checks the identity against the reference's matrix form. Per opposite pair
+// it is ~10 arithmetic instructions instead of a 19 x 19 transform.
 
 #include "common.h"
-#include "mrt_tables.h"
 
 namespace fsi {
 namespace {
-
-// relaxation class per moment, fsi/lbm3d.py:145-168: 0 conserved, 1 s_v, 2 s_o
-__host__ __device__ constexpr int rate_kind(int k) {
-  constexpr int t[19] = {0, 1, 1, 0, 2, 0, 2, 0, 2, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2};
-  return t[k];
-}
 
 template <typename R> struct LbmK {
   int nx, ny, nz;
   int64_t N;
   R s_v, s_o;
-  R cv, co;     // 1 - s/2 for the two relaxation rates
+  R cv;         // 1 - s_v/2
+  R ke;         // (2 + c_v)/3: weight of u.F in the even forcing term
   R rho0;
-  R rho_base;   // rho0 for shifted storage, 0 for absolute storage
-  R scale, inv_scale;
+  R scale, inv_scale;  // FP16 storage factors; I16: the scale fsi_lbm_init writes with
   R uf[3];      // uniform force when no field is given
   int mode[2];  // x-face boundary modes (left, right)
   R bc_rho[2];
   R bc_vel[2][3];
+  float* ctrl;  // I16: control block of the adaptive scale
+  int epoch;    // I16: write epoch of this launch
+  int pass;     // I16: 0 normal, 1 measuring (no stores), 2 scaled from the measurement
 };
 
-template <typename R> LbmK<R> make_consts(const fsi_lbm_params* p, const fsi_lbm_force* f) {
+template <typename R> LbmK<R> make_consts(const fsi_lbm_params* p, const fsi_lbm_force* f,
+                                          const fsi_lbm_buffers* b = nullptr) {
   LbmK<R> c;
   c.nx = p->nx;
   c.ny = p->ny;
@@ -65,12 +69,14 @@ template <typename R> LbmK<R> make_consts(const fsi_lbm_params* p, const fsi_lbm
   c.s_v = R(p->s_v);
   c.s_o = R(p->s_o);
   c.cv = R(1) - R(0.5) * c.s_v;
-  c.co = R(1) - R(0.5) * c.s_o;
+  c.ke = (R(2) + c.cv) / R(3);
   c.rho0 = R(p->rho0);
-  const bool shifted = p->storage == FSI_POP_F16;
-  c.rho_base = shifted ? R(p->rho0) : R(0);
+  const bool shifted = p->storage == FSI_POP_F16 || p->storage == FSI_POP_I16;
   c.scale = shifted ? R(p->pop_scale) : R(1);
   c.inv_scale = shifted ? R(1.0 / p->pop_scale) : R(1);
+  c.ctrl = b ? b->store_ctrl : nullptr;
+  c.epoch = p->store_epoch;
+  c.pass = p->storage == FSI_POP_I16 ? p->store_pass : 0;
   for (int a = 0; a < 3; ++a) c.uf[a] = f ? R(f->uniform[a]) : R(0);
   c.mode[0] = p->left_mode;
   c.mode[1] = p->right_mode;
@@ -97,95 +103,155 @@ __device__ __forceinline__ R feq_dev(int q, R rho, R rho0, const R u[3]) {
   }
 }
 
-template <typename R> __device__ __forceinline__ R frac(long long num, long long den) {
-  return R(num) / R(den);
-}
-
-// MRT collision in place on d[19] (absolute or deviation populations), given
-// the opposite-pair sums ps[p] = d[2p+1] + d[2p+2] and differences pd[p].
-// cv/co are 1 - s_v/2 and 1 - s_o/2.
+// Two-relaxation-time form of the reference's MRT collision (file comment),
+// in place on d[19] (absolute populations, or deviations from w rho0: g and
+// the forcing are the same expressions in both, with sigma = sum d the
+// density part). ps/pd: opposite-pair sums d_q + d_opp(q) and differences
+// for q = 1, 3, ..., 17; j = sum e d; u the collision velocity; fo the force.
 template <typename R>
-__device__ __forceinline__ void mrt_collide(R d[19], const R ps[9], const R pd[9], const R u[3],
-                                            const R fo[3], R s_v, R s_o, R cv, R co) {
-  using namespace fsi_mrt;
-  const R uf = u[0] * fo[0] + u[1] * fo[1] + u[2] * fo[2];
+__device__ __forceinline__ void trt_collide(R d[19], const R ps[9], const R pd[9], R sigma, const R j[3],
+                                            const R u[3], const R fo[3], const LbmK<R>& c) {
   const R uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-  R r[19];
-#pragma unroll
-  for (int k = 0; k < 19; ++k) {
-    // closed-form Guo moment G_k
-    R g = R(0);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (GA_NUM[k * 3 + a] != 0) g += frac<R>(GA_NUM[k * 3 + a], GA_DEN[k * 3 + a]) * fo[a];
-    if (GB_NUM[k] != 0) g -= frac<R>(GB_NUM[k], GB_DEN[k]) * uf;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        if (GC_NUM[(k * 3 + a) * 3 + b] != 0)
-          g += frac<R>(GC_NUM[(k * 3 + a) * 3 + b], GC_DEN[(k * 3 + a) * 3 + b]) * (u[a] * fo[b]);
-    if (rate_kind(k) == 0) {
-      r[k] = g;
-      continue;
-    }
-    // moment m_k from pair sums (even rows) or differences (odd rows)
-    R m = R(0);
-    if (kOdd[k]) {
-#pragma unroll
-      for (int p = 0; p < 9; ++p)
-        if (M[k][2 * p + 1] != 0) m += R(M[k][2 * p + 1]) * pd[p];
-    } else {
-      if (M[k][0] != 0) m += R(M[k][0]) * d[0];
-#pragma unroll
-      for (int p = 0; p < 9; ++p)
-        if (M[k][2 * p + 1] != 0) m += R(M[k][2 * p + 1]) * ps[p];
-    }
-    // equilibrium moment, fsi/lbm3d.py:285-312
-    R me = R(0);
-    switch (k) {
-      case 1: me = uu; break;
-      case 9: me = R(2) * u[0] * u[0] - u[1] * u[1] - u[2] * u[2]; break;
-      case 11: me = u[1] * u[1] - u[2] * u[2]; break;
-      case 13: me = u[0] * u[1]; break;
-      case 14: me = u[1] * u[2]; break;
-      case 15: me = u[0] * u[2]; break;
-      default: break;
-    }
-    const bool sv = rate_kind(k) == 1;
-    r[k] = (sv ? cv : co) * g - (sv ? s_v : s_o) * (m - me);
-  }
-  // M^-1 r = M^T (M M^T)^-1 r: block-diagonal Gram inverse, then integer M^T
-  R y[19];
-#pragma unroll
-  for (int a = 0; a < 19; ++a) {
-    R acc = R(0);
-#pragma unroll
-    for (int b = 0; b < 19; ++b)
-      if (GINV_NUM[a][b] != 0) acc += frac<R>(GINV_NUM[a][b], GINV_DEN[a][b]) * r[b];
-    y[a] = acc;
-  }
-  {
-    R e0 = R(0);
-#pragma unroll
-    for (int k = 0; k < 19; ++k)
-      if (!kOdd[k] && M[k][0] != 0) e0 += R(M[k][0]) * y[k];
-    d[0] += e0;
-  }
+  const R uf = u[0] * fo[0] + u[1] * fo[1] + u[2] * fo[2];
+  // w * base = s_v g^+(e.u = 0) - w ke u.F: the part of the even increment
+  // common to a weight class
+  const R base = c.s_v * (sigma - R(1.5) * uu) - c.ke * uf;
+  const R hv = R(0.5) * c.s_v, ho = R(0.5) * c.s_o;
+  d[0] += wq<R>(0) * base - c.s_v * d[0];
 #pragma unroll
   for (int p = 0; p < 9; ++p) {
     const int q = 2 * p + 1;
-    R ev = R(0), od = R(0);
-#pragma unroll
-    for (int k = 0; k < 19; ++k) {
-      if (M[k][q] == 0) continue;
-      if (kOdd[k])
-        od += R(M[k][q]) * y[k];
-      else
-        ev += R(M[k][q]) * y[k];
+    const R w = wq<R>(q);
+    R eu = R(0), ej = R(0), ef = R(0);
+    if (ex(q)) { eu += R(ex(q)) * u[0]; ej += R(ex(q)) * j[0]; ef += R(ex(q)) * fo[0]; }
+    if (ey(q)) { eu += R(ey(q)) * u[1]; ej += R(ey(q)) * j[1]; ef += R(ey(q)) * fo[1]; }
+    if (ez(q)) { eu += R(ez(q)) * u[2]; ej += R(ez(q)) * j[2]; ef += R(ez(q)) * fo[2]; }
+    const R even = w * base - hv * ps[p] + (w * eu) * (R(4.5) * c.s_v * eu + c.cv * ef);
+    const R odd = w * (R(3) * c.s_o * ej + ef) - ho * pd[p];
+    d[q] += even + odd;
+    d[q + 1] += even - odd;
+  }
+}
+
+// I16 storage: writes that record their largest value. Every one of the
+// first 64 (a flow starting from rest can grow fast), then every fourth (a
+// developed flow changes slowly; the maximum costs ~50 instructions of ~600
+// per wavefront). An untracked write leaves its maxima at zero, and the next
+// write keeps the scale.
+__device__ __forceinline__ bool i16_tracked(int epoch) { return epoch < 64 || epoch % 4 == 0; }
+
+// I16 storage: the scale of this launch's writes from the previous write's
+// scale and largest normalised value (every workgroup computes the same
+// value). The first writes aim low (a flow starting from rest can double
+// per step), later ones at 0.6 of the range, each rescale when the largest
+// value leaves [lo, hi]; at most x8 per step upwards.
+__device__ __forceinline__ float i16_next_scale(float s_in, float a, int epoch) {
+  const bool early = epoch < 16;
+  const float lo = early ? 0.1f : 0.4f, hi = early ? 0.5f : 0.8f, target = early ? 0.25f : 0.6f;
+  if (!(a > 0.0f)) return s_in;  // nothing stored yet (or a NaN): keep
+  // (1-ulp reciprocals: a scale only has to be the same where it is written
+  // and where it is read, and both take it from the control block)
+  const float r = target * __builtin_amdgcn_rcpf(a);
+  if (a > hi) return fmaxf(s_in * r, 1.0e-6f);
+  if (a < lo) return fminf(s_in * fminf(r, 8.0f), 1.0e30f);
+  return s_in;
+}
+
+// Largest of a non-negative float over the 64 lanes (all active), the same
+// value in every lane: DPP row shifts make lane 15 of each row the row's
+// maximum, two row broadcasts carry rows 0-2 into lane 63, which is read.
+// Non-negative floats order as their bit patterns.
+template <int CTRL, int ROW_MASK> __device__ __forceinline__ int dpp_max_step(int b) {
+  const int o = __builtin_amdgcn_update_dpp(b, b, CTRL, ROW_MASK, 0xf, false);
+  return (int)max((uint32_t)b, (uint32_t)o);
+}
+
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  int b = (int)__float_as_uint(v);
+  b = dpp_max_step<0x111, 0xf>(b);  // row_shr:1
+  b = dpp_max_step<0x112, 0xf>(b);  // row_shr:2
+  b = dpp_max_step<0x114, 0xf>(b);  // row_shr:4
+  b = dpp_max_step<0x118, 0xf>(b);  // row_shr:8
+  b = dpp_max_step<0x142, 0xa>(b);  // row_bcast:15 into rows 1 and 3
+  b = dpp_max_step<0x143, 0xc>(b);  // row_bcast:31 into rows 2 and 3
+  return __uint_as_float((uint32_t)__builtin_amdgcn_readlane(b, 63));
+}
+
+// Storage factors of one launch per weight class (rest, axis, diagonal): ld
+// turns a stored value of the previous write into storage units, st a value
+// in storage units into the stored representation of this write.
+template <typename S> struct Factors {
+  using R = typename PopStore<S>::real;
+  R ld[3], st[3];
+};
+
+template <typename S>
+__device__ __forceinline__ Factors<S> launch_factors(const LbmK<typename PopStore<S>::real>& c, bool init) {
+  using R = typename PopStore<S>::real;
+  Factors<S> f;
+  if constexpr (PopStore<S>::adaptive) {
+    const int e = c.epoch;
+    float s_in, s_out;
+    const int lane = threadIdx.x & 63;
+    if (init) {
+      s_in = s_out = (float)c.scale;
+    } else if (c.pass == 1) {
+      // measuring pass: values at the previous write's scale, nothing stored
+      s_in = s_out = c.ctrl[kCtrlSlot * ((e + 2) % 3)];
+    } else if (c.pass == 2) {
+      // the largest value this step stores, measured: put it at 0.8 of the range
+      const float* in = c.ctrl + kCtrlSlot * ((e + 2) % 3);
+      s_in = in[0];
+      const float a = wave_max_nonneg(c.ctrl[kCtrlSlot * kCtrlMeasure + kCtrlLine * (1 + (lane % kCtrlParts))]);
+      s_out = a > 0.0f ? fminf(s_in * 0.8f * __builtin_amdgcn_rcpf(a), 1.0e30f) : s_in;
+    } else {
+      const float* in = c.ctrl + kCtrlSlot * ((e + 2) % 3);
+      s_in = in[0];
+      // the previous write recorded its maximum only if it was tracked
+      const float a = i16_tracked(e - 1) ? wave_max_nonneg(in[kCtrlLine * (1 + (lane % kCtrlParts))]) : 0.0f;
+      s_out = i16_next_scale(s_in, a, e);
     }
-    d[q] += ev + od;
-    d[q + 1] += ev - od;
+    if (blockIdx.x == 0 && c.pass != 1) {
+      // this write's scale, and the next write's slot cleared for its maxima
+      // when that write is tracked (the last tracked write's maxima stay
+      // readable: its slot is next cleared three tracked writes later)
+      if (threadIdx.x == 0) c.ctrl[kCtrlSlot * (e % 3)] = s_out;
+      float* nxt = c.ctrl + kCtrlSlot * ((e + 1) % 3);
+      if (i16_tracked(e + 1) && threadIdx.x < kCtrlParts) nxt[kCtrlLine * (1 + threadIdx.x)] = 0.0f;
+    }
+    const float inv_in = __builtin_amdgcn_rcpf(32767.0f * s_in);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float w = wq<float>(k == 0 ? 0 : (k == 1 ? 1 : 7));
+      f.ld[k] = w * inv_in;
+      f.st[k] = s_out * float(wden(k == 0 ? 0 : (k == 1 ? 1 : 7)));  // s / w, w = 1 / wden
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      f.ld[k] = c.inv_scale;
+      f.st[k] = c.scale;
+    }
+  }
+  return f;
+}
+
+// Adds the workgroup's largest |stored value| (fraction of the 16-bit range)
+// to this write's maxima: one atomic per workgroup, spread over kCtrlParts
+// lines. Every thread of the workgroup calls it.
+template <int BLOCK>
+__device__ __forceinline__ void record_amax(float* ctrl, int slot, float amax) {
+  __shared__ uint32_t red[BLOCK / 64];
+  const float m = wave_max_nonneg(amax);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = __float_as_uint(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t b = red[0];
+#pragma unroll
+    for (int i = 1; i < BLOCK / 64; ++i) b = max(b, red[i]);
+    float* out = ctrl + kCtrlSlot * slot + kCtrlLine * (1 + blockIdx.x % kCtrlParts);
+    atomicMax(reinterpret_cast<unsigned int*>(out), b);
   }
 }
 
@@ -249,6 +315,8 @@ __global__ __launch_bounds__(kBlock) void k_lbm_init(LbmK<typename PopStore<S>::
                                                      typename PopStore<S>::real u0y,
                                                      typename PopStore<S>::real u0z) {
   using R = typename PopStore<S>::real;
+  // (I16: writes this epoch's scale before any thread returns)
+  const Factors<S> fac = launch_factors<S>(c, true);
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= c.N) return;
   const int k = (int)(n % c.nz);
@@ -277,22 +345,13 @@ __global__ __launch_bounds__(kBlock) void k_lbm_init(LbmK<typename PopStore<S>::
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     const R v = feq_dev<R, PopStore<S>::shifted>(q, c.rho0, c.rho0, u);
-    const S s = PopStore<S>::store(v, c.scale);
+    const S s = PopStore<S>::store(PopStore<S>::scaled(v, fac.st[wclass(q)]));
     pa[q * c.N + n] = s;
     pb[q * c.N + n] = s;
   }
 }
 
 enum { PHASE_STREAM_COLLIDE = 0, PHASE_COLLIDE = 1, PHASE_STREAM_MACRO = 2 };
-
-// Rest population of a solid cell in storage units: w_q rho0 (absolute) or 0
-// (shifted). Solid cells keep it for ever (the reference never updates them).
-template <typename S>
-__device__ __forceinline__ S solid_value(int q, typename PopStore<S>::real rho0,
-                                         typename PopStore<S>::real scale) {
-  using R = typename PopStore<S>::real;
-  return PopStore<S>::store(PopStore<S>::shifted ? R(0) : wq<R>(q) * rho0, scale);
-}
 
 // x-face boundary of one face cell on its streamed populations d[19]
 // (fsi/lbm3d.py:363-407). prev_u: the face cell's macroscopic velocity of the
@@ -322,7 +381,7 @@ __device__ __forceinline__ void x_face_boundary(R d[19], int mode, R rho_bc, con
 // amortise that for FP32 (measured 2% faster at 256^3), while FP16 runs best
 // at 256.
 template <typename S> struct PullBlock { static constexpr int value = 256; };
-template <> struct PullBlock<float> { static constexpr int value = 1024; };
+template <> struct PullBlock<float> { static constexpr int value = 512; };
 
 template <typename S, int PHASE, bool FORCE_FIELD, bool FACES>
 __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
@@ -332,10 +391,28 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
     const typename PopStore<S>::real* __restrict__ fv_prev,
     typename PopStore<S>::real* __restrict__ fv_next) {
   using R = typename PopStore<S>::real;
-  constexpr bool SH = PopStore<S>::shifted;
+  // the arithmetic runs on deviations d = F - w rho0 for every storage: the
+  // 16-bit storages keep them, the absolute FP32/FP64 populations subtract
+  // and add back w rho0 at the load and the store (exact for F within a
+  // factor 2 of w rho0), so the collision's FP32 rounding is relative to the
+  // deviations, not to the ~w-sized populations
+  constexpr bool ABS = !PopStore<S>::shifted;
+  constexpr bool ADAPT = PopStore<S>::adaptive;
+  const Factors<S> fac = launch_factors<S>(c, false);
+  R base[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) base[k] = ABS ? wq<R>(k == 0 ? 0 : (k == 1 ? 1 : 7)) * c.rho0 : R(0);
   const uint32_t N = (uint32_t)c.N;
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+  const uint32_t n_lane = blockIdx.x * blockDim.x + threadIdx.x;
+  // adaptive storage: every thread reaches the workgroup's maximum at the
+  // end, so lanes past the lattice compute the last cell and store nothing
+  const bool live = n_lane < N;
+  if constexpr (!ADAPT) {
+    if (!live) return;
+  }
+  // a measuring pass computes the step and its largest stored value only
+  const bool write = live && (!ADAPT || c.pass != 1);
+  const uint32_t n = live ? n_lane : N - 1;
   constexpr uint32_t B = sizeof(S);
   const uint32_t plane = N * B;  // byte size of one population plane
   const uint32_t own = n * B;
@@ -348,10 +425,14 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   const uint32_t j = ij % ny;
   const uint32_t i = ij / ny;
 
-  R d[19];
+  // 16-bit storages: all 19 loads are issued before any is used (a barrier
+  // to the instruction scheduler: interleaving each conversion with its load
+  // put a wait for every second load and cost ~25% of the step; FP32 runs
+  // ~3% faster without the barrier)
+  S raw[19];
   if constexpr (PHASE == PHASE_COLLIDE) {
 #pragma unroll
-    for (int q = 0; q < 19; ++q) d[q] = PopStore<S>::load(ld_off(pin, q * plane + own), c.inv_scale);
+    for (int q = 0; q < 19; ++q) raw[q] = ld_off(pin, q * plane + own);
   } else {
     // gather f*(c - e_q, q), or f*(c, opp q) when c - e_q = c + e_opp(q) is
     // solid (fsi/lbm3d.py:352-361 read backwards). One-cell periodic wrap on
@@ -379,9 +460,13 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
       const uint32_t a0 = q * plane + src * B;
       const uint32_t a1 = opp(q) * plane + own;
       const uint32_t sel = 0u - ((all >> opp(q)) & 1u);
-      d[q] = PopStore<S>::load(ld_off(pin, a0 ^ ((a0 ^ a1) & sel)), c.inv_scale);
+      raw[q] = ld_off(pin, a0 ^ ((a0 ^ a1) & sel));
     }
   }
+  if constexpr (sizeof(S) == 2) __builtin_amdgcn_sched_barrier(0);
+  R d[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) d[q] = PopStore<S>::load(raw[q], fac.ld[wclass(q)]) - base[wclass(q)];
 
   R fo[3];
   if constexpr (FORCE_FIELD) {
@@ -408,37 +493,34 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
       const R* fp = fv_prev + side * 3 * fplane;
       const R pu[3] = {fp[jk], fp[fplane + jk], fp[2 * fplane + jk]};
       const bool inner_solid = (mask >> (side == 0 ? 1 : 2)) & 1u;
-      x_face_boundary<R, SH>(d, c.mode[side], c.bc_rho[side], c.bc_vel[side], c.rho0, pu, inner_solid);
+      x_face_boundary<R, true>(d, c.mode[side], c.bc_rho[side], c.bc_vel[side], c.rho0, pu, inner_solid);
     }
   }
 
+  // opposite-pair sums and differences; density part and momentum of d
   R ps[9], pd[9];
+  R sigma = d[0];
+  R jm[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int p = 0; p < 9; ++p) {
-    ps[p] = d[2 * p + 1] + d[2 * p + 2];
-    pd[p] = d[2 * p + 1] - d[2 * p + 2];
+    const int q = 2 * p + 1;
+    ps[p] = d[q] + d[q + 1];
+    pd[p] = d[q] - d[q + 1];
+    sigma += ps[p];
+    if (ex(q)) jm[0] += R(ex(q)) * pd[p];
+    if (ey(q)) jm[1] += R(ey(q)) * pd[p];
+    if (ez(q)) jm[2] += R(ez(q)) * pd[p];
   }
 
-  // macroscopic state (fsi/lbm3d.py:409-425): stored, or from the streamed
-  // populations with the force the previous collision used
-  R rho, u[3];
+  // macroscopic velocity (fsi/lbm3d.py:409-425): stored, or from the
+  // streamed populations with the force the previous collision used
+  R u[3];
   if constexpr (PHASE == PHASE_COLLIDE) {
-    rho = rho_io[n];
     u[0] = vel_io[n];
     u[1] = vel_io[N + n];
     u[2] = vel_io[2 * N + n];
   } else {
-    R sum = d[0];
-    R jm[3] = {R(0), R(0), R(0)};
-#pragma unroll
-    for (int p = 0; p < 9; ++p) {
-      sum += ps[p];
-      const int q = 2 * p + 1;
-      if (ex(q)) jm[0] += R(ex(q)) * pd[p];
-      if (ey(q)) jm[1] += R(ey(q)) * pd[p];
-      if (ez(q)) jm[2] += R(ez(q)) * pd[p];
-    }
-    rho = c.rho_base + sum;
+    R rho = c.rho0 + sigma;
     const R inv = recip(rho);
 #pragma unroll
     for (int a = 0; a < 3; ++a) u[a] = (jm[a] + fo[a] * R(0.5)) * inv;
@@ -446,34 +528,53 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
       rho = c.rho0;
       u[0] = u[1] = u[2] = R(0);
     }
-    if constexpr (FACES) {
-      if (face) {
-        R* fn = fv_next + side * 3 * fplane;
-        fn[jk] = u[0];
-        fn[fplane + jk] = u[1];
-        fn[2 * fplane + jk] = u[2];
+    if (write) {
+      if constexpr (FACES) {
+        if (face) {
+          R* fn = fv_next + side * 3 * fplane;
+          fn[jk] = u[0];
+          fn[fplane + jk] = u[1];
+          fn[2 * fplane + jk] = u[2];
+        }
+      }
+      if constexpr (PHASE == PHASE_STREAM_MACRO) {
+        rho_io[n] = rho;
+        vel_io[n] = u[0];
+        vel_io[N + n] = u[1];
+        vel_io[2 * N + n] = u[2];
       }
     }
   }
 
-  if constexpr (PHASE == PHASE_STREAM_MACRO) {
-    rho_io[n] = rho;
-    vel_io[n] = u[0];
-    vel_io[N + n] = u[1];
-    vel_io[2 * N + n] = u[2];
+  if constexpr (PHASE != PHASE_STREAM_MACRO) trt_collide<R>(d, ps, pd, sigma, jm, u, fo, c);
+  // store the post-collision (or, for STREAM_MACRO, post-stream) populations;
+  // solid cells store their rest value, deviation 0 (a zero store factor in
+  // the 16-bit storages, where it saves a select per population)
+  R st[3];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const S v = solid ? solid_value<S>(q, c.rho0, c.scale) : PopStore<S>::store(d[q], c.scale);
-      st_stream(pout, q * plane + own, v);
-    }
-    return;
-  }
-
-  mrt_collide<R>(d, ps, pd, u, fo, c.s_v, c.s_o, c.cv, c.co);
+  for (int k = 0; k < 3; ++k) st[k] = (!ABS && solid) ? R(0) : fac.st[k];
+  R x[19];
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const S v = solid ? solid_value<S>(q, c.rho0, c.scale) : PopStore<S>::store(d[q], c.scale);
-    st_stream(pout, q * plane + own, v);
+    if constexpr (ABS)
+      x[q] = (solid ? R(0) : d[q]) + base[wclass(q)];
+    else
+      x[q] = PopStore<S>::scaled(d[q], st[wclass(q)]);
+  }
+  if (write) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) st_stream(pout, q * plane + own, PopStore<S>::store(x[q]));
+  }
+  if constexpr (ADAPT) {
+    if (c.pass == 1 || i16_tracked(c.epoch)) {  // uniform
+      // largest |x| from the bit patterns (no NaN canonicalisation per value)
+      uint32_t m = __float_as_uint((float)x[0]) & 0x7fffffffu;
+#pragma unroll
+      for (int q = 1; q < 19; q += 2)
+        m = max(m, max(__float_as_uint((float)x[q]) & 0x7fffffffu, __float_as_uint((float)x[q + 1]) & 0x7fffffffu));
+      record_amax<PullBlock<S>::value>(c.ctrl, c.pass == 1 ? kCtrlMeasure : c.epoch % 3,
+                                       live ? __uint_as_float(m) : 0.0f);
+    }
   }
 }
 
@@ -517,9 +618,20 @@ __global__ __launch_bounds__(kBlock) void k_export_pop(LbmK<typename PopStore<S>
   using R = typename PopStore<S>::real;
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= c.N) return;
+  R ld[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if constexpr (PopStore<S>::adaptive) {
+      // the scale of the current populations (write epoch c.epoch)
+      const float w = wq<float>(k == 0 ? 0 : (k == 1 ? 1 : 7));
+      ld[k] = w * __builtin_amdgcn_rcpf(32767.0f * c.ctrl[kCtrlSlot * (c.epoch % 3)]);
+    } else {
+      ld[k] = c.inv_scale;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const R dq = PopStore<S>::load(pop[q * c.N + n], c.inv_scale);
+    const R dq = PopStore<S>::load(pop[q * c.N + n], ld[wclass(q)]);
     out[n * 19 + q] = PopStore<S>::shifted ? wq<R>(q) * c.rho0 + dq : dq;
   }
 }
@@ -560,8 +672,11 @@ int check_params(const fsi_lbm_params* p) {
   FSI_REQUIRE(p != nullptr, "lbm params are null");
   FSI_REQUIRE(p->nx > 4 && p->ny > 4 && p->nz > 4, "lattice dims must exceed 4, got %d %d %d",
               p->nx, p->ny, p->nz);
-  FSI_REQUIRE(p->storage == FSI_POP_F32 || p->storage == FSI_POP_F16 || p->storage == FSI_POP_F64,
+  FSI_REQUIRE(p->storage == FSI_POP_F32 || p->storage == FSI_POP_F16 || p->storage == FSI_POP_F64 ||
+                  p->storage == FSI_POP_I16,
               "unknown population storage %d", p->storage);
+  FSI_REQUIRE(p->storage != FSI_POP_I16 || (p->pop_scale > 0.0 && p->store_epoch >= 0),
+              "I16 storage needs a positive initial scale and a write epoch");
   FSI_REQUIRE(p->left_mode >= 0 && p->left_mode <= 2 && p->right_mode >= 0 && p->right_mode <= 2,
               "unknown boundary mode");
   return 0;
@@ -571,7 +686,7 @@ template <typename S>
 int lbm_init_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pa, void* pb,
                hipStream_t st) {
   using R = typename PopStore<S>::real;
-  LbmK<R> c = make_consts<R>(p, nullptr);
+  LbmK<R> c = make_consts<R>(p, nullptr, b);
   hipLaunchKernelGGL(k_lbm_init<S>, dim3(blocks_for(c.N)), dim3(kBlock), 0, st, c, b->solid,
                      b->nbmask, (R*)b->rho, (R*)b->vel, (S*)pa, (S*)pb, (R*)b->face_vel,
                      R(p->initial_velocity[0]), R(p->initial_velocity[1]),
@@ -600,7 +715,7 @@ template <typename S>
 int lbm_step_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pin, void* pout,
                const fsi_lbm_force* f, int phase, int face_slot, hipStream_t st) {
   using R = typename PopStore<S>::real;
-  LbmK<R> c = make_consts<R>(p, f);
+  LbmK<R> c = make_consts<R>(p, f, b);
   FSI_REQUIRE(19.0 * (double)c.N * sizeof(S) < 4294967296.0,
               "lbm_step: lattice of %lld cells too large for 32-bit plane offsets", (long long)c.N);
   const bool faces = p->left_mode != FSI_BC_PERIODIC || p->right_mode != FSI_BC_PERIODIC;
@@ -631,9 +746,10 @@ int lbm_step_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pi
 }
 
 template <typename S>
-int lbm_export_t(const fsi_lbm_params* p, const void* pop, void* out, hipStream_t st) {
+int lbm_export_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop, void* out,
+                 hipStream_t st) {
   using R = typename PopStore<S>::real;
-  LbmK<R> c = make_consts<R>(p, nullptr);
+  LbmK<R> c = make_consts<R>(p, nullptr, b);
   hipLaunchKernelGGL(k_export_pop<S>, dim3(blocks_for(c.N)), dim3(kBlock), 0, st, c,
                      (const S*)pop, (R*)out);
   FSI_TRY_LAUNCH("lbm_export_populations");
@@ -652,10 +768,12 @@ int fsi_lbm_init(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pop_a,
   if (int rc = check_params(p)) return rc;
   FSI_REQUIRE(b && b->solid && b->nbmask && b->rho && b->vel && pop_a && pop_b,
               "lbm_init: null buffer");
+  FSI_REQUIRE(p->storage != FSI_POP_I16 || b->store_ctrl, "lbm_init: I16 storage needs store_ctrl");
   hipStream_t st = (hipStream_t)stream;
   switch (p->storage) {
     case FSI_POP_F32: return lbm_init_t<float>(p, b, pop_a, pop_b, st);
     case FSI_POP_F16: return lbm_init_t<__half>(p, b, pop_a, pop_b, st);
+    case FSI_POP_I16: return lbm_init_t<int16_t>(p, b, pop_a, pop_b, st);
     default: return lbm_init_t<double>(p, b, pop_a, pop_b, st);
   }
 }
@@ -666,10 +784,12 @@ int fsi_lbm_step(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* 
   if (int rc = check_params(p)) return rc;
   FSI_REQUIRE(b && b->nbmask && pop_in && pop_out && pop_in != pop_out,
               "lbm_step: null or aliased buffer");
+  FSI_REQUIRE(p->storage != FSI_POP_I16 || b->store_ctrl, "lbm_step: I16 storage needs store_ctrl");
   hipStream_t st = (hipStream_t)stream;
   switch (p->storage) {
     case FSI_POP_F32: return lbm_step_t<float>(p, b, pop_in, pop_out, force, phase, face_slot, st);
     case FSI_POP_F16: return lbm_step_t<__half>(p, b, pop_in, pop_out, force, phase, face_slot, st);
+    case FSI_POP_I16: return lbm_step_t<int16_t>(p, b, pop_in, pop_out, force, phase, face_slot, st);
     default: return lbm_step_t<double>(p, b, pop_in, pop_out, force, phase, face_slot, st);
   }
 }
@@ -711,15 +831,18 @@ int fsi_lbm_force_import(const fsi_lbm_params* p, const fsi_lbm_buffers* b, cons
   return 0;
 }
 
-int fsi_lbm_export_populations(const fsi_lbm_params* p, const void* pop, void* out_aos,
-                               void* stream) {
+int fsi_lbm_export_populations(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pop,
+                               void* out_aos, void* stream) {
   if (int rc = check_params(p)) return rc;
   FSI_REQUIRE(pop && out_aos, "lbm_export_populations: null buffer");
+  FSI_REQUIRE(p->storage != FSI_POP_I16 || (b && b->store_ctrl),
+              "lbm_export_populations: I16 storage needs store_ctrl");
   hipStream_t st = (hipStream_t)stream;
   switch (p->storage) {
-    case FSI_POP_F32: return lbm_export_t<float>(p, pop, out_aos, st);
-    case FSI_POP_F16: return lbm_export_t<__half>(p, pop, out_aos, st);
-    default: return lbm_export_t<double>(p, pop, out_aos, st);
+    case FSI_POP_F32: return lbm_export_t<float>(p, b, pop, out_aos, st);
+    case FSI_POP_F16: return lbm_export_t<__half>(p, b, pop, out_aos, st);
+    case FSI_POP_I16: return lbm_export_t<int16_t>(p, b, pop, out_aos, st);
+    default: return lbm_export_t<double>(p, b, pop, out_aos, st);
   }
 }
 

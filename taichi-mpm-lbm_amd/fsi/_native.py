@@ -15,11 +15,16 @@ import torch
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libfsi_hip.so"
 _lib = None
 
-POP_F32, POP_F16, POP_F64 = 0, 1, 2
+POP_F32, POP_F16, POP_F64, POP_I16 = 0, 1, 2, 3
 PHASE_STREAM_COLLIDE, PHASE_COLLIDE, PHASE_STREAM_MACRO = 0, 1, 2
 REAL_F32, REAL_F64 = 0, 1
 BC_MODES = {"periodic": 0, "pressure": 1, "velocity": 2}
-STORAGE_CODES = {"fp32": POP_F32, "fp16": POP_F16, "fp64": POP_F64}
+STORAGE_CODES = {"fp32": POP_F32, "fp16": POP_F16, "fp64": POP_F64, "i16": POP_I16}
+# floats of the I16 storage control block (FSI_STORE_CTRL_FLOATS), and the
+# layout the host reads it with: four slots (three write epochs, one
+# measuring pass) of 65 lines of 32 floats, the scale at the start of a slot,
+# the 64 partial maxima one per following line
+STORE_CTRL_FLOATS, STORE_CTRL_SLOT, STORE_CTRL_LINE = 8320, 2080, 32
 
 c_double3 = ctypes.c_double * 3
 _vp = ctypes.c_void_p
@@ -36,12 +41,14 @@ class LbmParams(ctypes.Structure):
         ("bc_rho", ctypes.c_double * 2),
         ("bc_vel", c_double3 * 2),
         ("pop_scale", ctypes.c_double),
+        ("store_epoch", ctypes.c_int32),
+        ("store_pass", ctypes.c_int32),
     ]
 
 
 class LbmBuffers(ctypes.Structure):
     _fields_ = [("solid", _vp), ("nbmask", _vp), ("rho", _vp), ("vel", _vp), ("force", _vp),
-                ("face_vel", _vp)]
+                ("face_vel", _vp), ("store_ctrl", _vp)]
 
 
 class LbmForce(ctypes.Structure):
@@ -104,7 +111,7 @@ _SIGNATURES = {
     "fsi_lbm_step": [_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp],
     "fsi_lbm_force_op": [_vp, _vp, ctypes.c_int32, _vp, _vp, _vp, _vp],
     "fsi_lbm_force_import": [_vp, _vp, _vp, _vp],
-    "fsi_lbm_export_populations": [_vp, _vp, _vp, _vp],
+    "fsi_lbm_export_populations": [_vp, _vp, _vp, _vp, _vp],
     "fsi_soa_to_aos": [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp],
     "fsi_aos_to_soa": [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp],
     "fsi_mpm_clear_grid": [_vp, _vp, _vp],

@@ -4,7 +4,8 @@ Same classes, field names, defaults and validation rules as the reference's
 ``fsi/config.py:20-259`` so existing scene scripts construct the same objects.
 Two build-only fields are added with defaults that keep the reference
 behaviour: ``LBMConfig.storage`` selects the population storage format
-("fp32" as in the reference, "fp16" shifted/scaled storage, "fp64" the
+("fp32" as in the reference, "fp16" shifted/scaled IEEE half storage, "i16"
+shifted signed-normalised 16-bit storage with an adaptive scale, "fp64" the
 double-precision build), and ``MPMConfig``/``CouplingConfig`` are unchanged.
 """
 
@@ -18,14 +19,14 @@ Backend = Literal["cpu", "gpu", "cuda", "vulkan"]
 BoundaryType = Literal["periodic", "pressure", "velocity", "wall"]
 CouplingKernel = Literal["quadratic"]
 OutputFormat = Literal["npz", "vtk", "both"]
-PopulationStorage = Literal["fp32", "fp16", "fp64"]
+PopulationStorage = Literal["fp32", "fp16", "i16", "fp64"]
 
 _BACKENDS = ("cpu", "gpu", "cuda", "vulkan")
 _BOUNDARY_TYPES = ("periodic", "pressure", "velocity", "wall")
 _COUPLING_KERNELS = ("quadratic",)
 _OUTPUT_FORMATS = ("npz", "vtk", "both")
 _MATERIALS = ("elastic",)
-_STORAGES = ("fp32", "fp16", "fp64")
+_STORAGES = ("fp32", "fp16", "i16", "fp64")
 
 
 def _require(condition: bool, message: str) -> None:
@@ -71,7 +72,8 @@ class LBMConfig:
 
     use_sparse: bool = False
     # population storage on the device: "fp32" (reference arithmetic),
-    # "fp16" (f - w*rho0 scaled by a power of two, FP32 collision) or "fp64"
+    # "fp16" (f - w*rho0 scaled by a power of two, FP32 collision), "i16"
+    # ((f - w*rho0)/w as 16-bit integers of an adaptive range) or "fp64"
     storage: PopulationStorage = "fp32"
 
     def validate(self) -> None:

@@ -18,7 +18,10 @@ x-face boundaries, macroscopic fields, collision). Anything that needs the
 reference's post-step state - the accessors, the coupler, and any change of
 the force field (the macroscopic velocity must use the force of the step
 that produced it) - first materialises it; the next step then only collides.
-FP16 storage keeps (f - w rho0) * pop_scale.
+FP16 storage keeps (f - w rho0) * pop_scale; I16 storage keeps
+(f - w rho0) / w * s as 16-bit integers, the kernels adapting s to the
+largest stored value through a small device control block (the host only
+counts the population writes).
 
 When the host knows the force is uniform on fluid cells
 (``_force_uniform``), kernels take the value as a constant and read no force
@@ -77,13 +80,14 @@ class LBMSolver3D:
         self.bc_x_left_mode = _boundary_mode(config.x_left)
         self.bc_x_right_mode = _boundary_mode(config.x_right)
         self.storage = config.storage
-        self.pop_scale = POP_SCALE_FP16 if self.storage == "fp16" else 1.0
+        self.pop_scale = {"fp16": POP_SCALE_FP16, "i16": self._i16_initial_scale()}.get(self.storage, 1.0)
 
         self.device = nat.device()
         self.real_dtype = torch.float64 if self.storage == "fp64" else torch.float32
         self.real_code = nat.REAL_F64 if self.storage == "fp64" else nat.REAL_F32
         self._np_real = np.float64 if self.storage == "fp64" else np.float32
-        pop_dtype = {"fp32": torch.float32, "fp16": torch.float16, "fp64": torch.float64}[self.storage]
+        pop_dtype = {"fp32": torch.float32, "fp16": torch.float16, "i16": torch.int16,
+                     "fp64": torch.float64}[self.storage]
 
         n = self.n_cells
         dev, rd = self.device, self.real_dtype
@@ -97,11 +101,15 @@ class LBMSolver3D:
         # two slots (previous / next step) x two faces x 3 components
         self.face_vel = torch.zeros(12 * self.ny * self.nz, dtype=rd, device=dev)
         self._sum_ws, self._sum_out = nat.reduction_buffers(dev)
+        # I16 storage: scale and range of the last population writes (device)
+        self.store_ctrl = (torch.zeros(nat.STORE_CTRL_FLOATS, dtype=torch.float32, device=dev)
+                           if self.storage == "i16" else None)
+        self._epoch = 0  # population writes since initialize()
 
         self._params = self._make_params()
         self._buffers = nat.LbmBuffers(
             nat.ptr(self.solid), nat.ptr(self.nbmask), nat.ptr(self.rho), nat.ptr(self.vel),
-            nat.ptr(self.force), nat.ptr(self.face_vel),
+            nat.ptr(self.force), nat.ptr(self.face_vel), nat.ptr(self.store_ctrl),
         )
         self._materialized = False
         self._face_slot = 0
@@ -114,7 +122,7 @@ class LBMSolver3D:
     def _validate_supported_config(self) -> None:
         # fsi/lbm3d.py:85-108
         c = self.config
-        pop_bytes = {"fp32": 4, "fp16": 2, "fp64": 8}.get(getattr(c, "storage", "fp32"), 4)
+        pop_bytes = {"fp32": 4, "fp16": 2, "i16": 2, "fp64": 8}.get(getattr(c, "storage", "fp32"), 4)
         if _Q * int(c.nx) * int(c.ny) * int(c.nz) * pop_bytes >= 2 ** 32:
             # the lattice kernels address a population buffer with 32-bit
             # byte offsets (one VGPR per address, SGPR base)
@@ -156,7 +164,53 @@ class LBMSolver3D:
         p.bc_vel[0] = nat.c_double3(*[float(v) for v in c.x_left.velocity])
         p.bc_vel[1] = nat.c_double3(*[float(v) for v in c.x_right.velocity])
         p.pop_scale = float(self.pop_scale)
+        p.store_epoch = 0
         return p
+
+    def _i16_initial_scale(self) -> float:
+        """Scale of the I16 populations fsi_lbm_init writes: the largest
+        normalised deviation (f - w rho0)/w = (rho - rho0) + rho (3 e.u +
+        4.5 (e.u)^2 - 1.5 u.u) the configured velocities, face densities and
+        force give (|e.u| <= sqrt(2)|u|), at a quarter of the 16-bit range,
+        and no finer than for a deviation of 2.5e-3 (a flow the coupling
+        starts); the lattice kernels adapt it within the first steps."""
+        c = self.config
+        speeds = [float(np.linalg.norm(c.initial_velocity))]
+        drho = 0.0
+        for face in (c.x_left, c.x_right):
+            if face.boundary_type == "velocity":
+                speeds.append(float(np.linalg.norm(face.velocity)))
+            elif face.boundary_type == "pressure":
+                drho = max(drho, abs(float(face.rho) - float(c.rho0)))
+        v = max(speeds)
+        est = drho + (1.0 + drho) * (3.0 * np.sqrt(2.0) * v + 10.5 * v * v) + 3.0 * float(np.linalg.norm(c.force))
+        return 0.25 / max(est, 2.5e-3)
+
+    @staticmethod
+    def _i16_tracked(epoch: int) -> bool:
+        # the writes whose kernels record their largest value (lbm.hip)
+        return epoch < 64 or epoch % 4 == 0
+
+    def storage_range(self) -> float:
+        """I16 storage: the largest |stored value| of the last tracked
+        population write (every one of the first 64 writes, then every
+        fourth) as a fraction of the 16-bit range: above 1, values were
+        clipped; 0 right after initialize. 0 for the other storages."""
+        if self.store_ctrl is None:
+            return 0.0
+        t = self._epoch
+        while t > 0 and not self._i16_tracked(t):
+            t -= 1
+        slot = (t % 3) * nat.STORE_CTRL_SLOT
+        parts = self.store_ctrl[slot:slot + nat.STORE_CTRL_SLOT].view(-1, nat.STORE_CTRL_LINE)[1:, 0]
+        return float(parts.max().item())
+
+    def storage_scale(self) -> float:
+        """Scale s of the current I16 populations ((f - w rho0)/w * s is
+        stored as round(32767 x)); the fixed FP16 scale, 1 otherwise."""
+        if self.store_ctrl is None:
+            return float(self.pop_scale)
+        return float(self.store_ctrl[(self._epoch % 3) * nat.STORE_CTRL_SLOT].item())
 
     def initialize(self, solid_np: np.ndarray | None = None) -> None:
         # fsi/lbm3d.py:193-203
@@ -165,6 +219,10 @@ class LBMSolver3D:
         else:
             solid = self._normalize_solid_mask(solid_np)
         self.solid.copy_(torch.from_numpy(solid.reshape(-1)))
+        if self.store_ctrl is not None:
+            nat.fill_zero(self.store_ctrl)
+        self._epoch = 0
+        self._params.store_epoch = 0
         nat.call("fsi_lbm_init", nat.ref(self._params), nat.ref(self._buffers),
                  nat.ptr(self.pop[0]), nat.ptr(self.pop[1]))
         self._cur = 0
@@ -205,11 +263,25 @@ class LBMSolver3D:
             f.uniform = nat.c_double3(0.0, 0.0, 0.0)
         return f
 
+    # I16 storage: the first population writes after initialize() are
+    # measured first (a pass without stores), so their scale fits what they
+    # store whatever force or boundary starts the flow
+    _MEASURED_WRITES = 3
+
     def _launch(self, phase: int) -> None:
         f = self._force_arg()
-        nat.call("fsi_lbm_step", nat.ref(self._params), nat.ref(self._buffers),
-                 nat.ptr(self.pop[self._cur]), nat.ptr(self.pop[1 - self._cur]), nat.ref(f),
-                 phase, self._face_slot)
+        self._params.store_epoch = self._epoch + 1
+        args = (nat.ref(self._params), nat.ref(self._buffers), nat.ptr(self.pop[self._cur]),
+                nat.ptr(self.pop[1 - self._cur]), nat.ref(f), phase, self._face_slot)
+        if self.store_ctrl is not None and self._epoch < self._MEASURED_WRITES:
+            m = 3 * nat.STORE_CTRL_SLOT
+            nat.fill_zero(self.store_ctrl[m:m + nat.STORE_CTRL_SLOT])
+            self._params.store_pass = 1
+            nat.call("fsi_lbm_step", *args)
+            self._params.store_pass = 2
+        nat.call("fsi_lbm_step", *args)
+        self._params.store_pass = 0
+        self._epoch += 1
         self._cur = 1 - self._cur
         if phase != nat.PHASE_COLLIDE and self._faces:
             self._face_slot = 1 - self._face_slot
@@ -332,7 +404,9 @@ class LBMSolver3D:
     def distribution_numpy(self) -> np.ndarray:
         self._ensure_macro()
         out = torch.empty(self.n_cells * _Q, dtype=self.real_dtype, device=self.device)
-        nat.call("fsi_lbm_export_populations", nat.ref(self._params), nat.ptr(self.pop[self._cur]),
+        self._params.store_epoch = self._epoch
+        nat.call("fsi_lbm_export_populations", nat.ref(self._params), nat.ref(self._buffers),
+                 nat.ptr(self.pop[self._cur]),
                  nat.ptr(out))
         return out.cpu().numpy().reshape(self.nx, self.ny, self.nz, _Q)
 

@@ -11,6 +11,8 @@ with rel = max|a - b| / max|b|:
   makes against FP64, plus 1e-7 - i.e. as accurate as the reference.
 * FP16 shifted storage vs FP64 oracle: velocity rel <= 1e-3 (measured <= 3.8e-4 on these
   scenarios), density rel <= 1e-5.
+* I16 storage (16-bit integers of an adaptive range) vs FP64 oracle: velocity
+  rel <= 2e-4, density rel <= 1e-5, and no stored value clipped.
 """
 
 import dataclasses
@@ -116,16 +118,68 @@ def test_lbm_fp16_shifted_storage_error(name, oracle_runs):
     assert eu <= 1e-3 and er <= 1e-5, (name, eu, er)
 
 
+@pytest.mark.parametrize("name", list(SCENARIOS))
+def test_lbm_i16_storage_error(name, oracle_runs):
+    import fsi
+
+    sc = SCENARIOS[name]
+    cfg = dataclasses.replace(sc["cfg"], storage="i16")
+    s = run_lbm(fsi.LBMSolver3D, cfg, sc["steps"], solid=sc.get("solid"),
+                force_field=sc.get("force_field"), force_change=sc.get("force_change"))
+    got = lbm_fields(s, cfg.rho0)
+    ref = oracle_runs[name]["f64"]
+    eu = rel_inf(got["u"], ref["u"])
+    er = rel_inf(got["rho"], ref["rho"])
+    rng = s.storage_range()
+    print(f"{name} i16: u rel {eu:.3e} rho rel {er:.3e} range used {rng:.3f} scale {s.storage_scale():.3e}")
+    assert eu <= 2e-4 and er <= 1e-5, (name, eu, er)
+    assert 0.0 < rng <= 1.0
+
+
+def test_sixteen_bit_storages_over_400_steps():
+    """64^3 porous body-force lattice (porosity 0.8, spheres r 4-10, tau =
+    0.55, g_x = 1e-6 as the 256^3 bandwidth case) over 400 steps against the
+    FP64 build (pinned to the oracle above): the rounding of the stored
+    populations accumulates while the flow develops. Max-norm relative u:
+    I16 <= 1e-3 (it resolves each population to 2^-15 of the largest
+    normalised deviation); IEEE FP16 (11-bit significand) is only reported
+    against a looser 4e-3."""
+    import fsi
+    from fsi.scenes import porous_spheres
+
+    from bench import periodic_lbm_config
+
+    n, steps = 64, 400
+    mask, _ = porous_spheres((n, n, n), porosity=0.8, rmin=4.0, rmax=10.0, seed=0)
+    vel = {}
+    ranges = []
+    for storage in ("fp64", "fp16", "i16"):
+        s = fsi.LBMSolver3D(periodic_lbm_config(n, storage))
+        s.initialize(mask)
+        for _ in range(steps):
+            s.step()
+        vel[storage] = s.velocity_numpy().astype(np.float64)
+        if storage == "i16":
+            ranges.append(s.storage_range())
+    errs = {k: rel_inf(vel[k], vel["fp64"]) for k in ("fp16", "i16")}
+    print(f"400 steps 64^3: max|u| {np.abs(vel['fp64']).max():.3e}, u rel max fp16 {errs['fp16']:.3e} "
+          f"i16 {errs['i16']:.3e}, i16 range used {ranges[0]:.3f}")
+    assert errs["i16"] <= 1e-3, errs
+    assert errs["fp16"] <= 4e-3, errs
+    assert 0.0 < ranges[0] <= 1.0
+
+
 def test_solid_cells_keep_initial_populations_and_zero_velocity():
     import fsi
 
-    cfg, solid = channel_config()
-    s = run_lbm(fsi.LBMSolver3D, cfg, 10, solid=solid)
-    f = s.distribution_numpy()
-    assert np.array_equal(f[solid == 1], np.broadcast_to(np.float32(W * cfg.rho0), f[solid == 1].shape))
-    u = s.velocity_numpy()
-    assert np.all(u[solid == 1] == 0.0)
-    assert np.all(s.density_numpy()[solid == 1] == np.float32(cfg.rho0))
+    for storage in ("fp32", "fp16", "i16"):
+        cfg, solid = channel_config(storage=storage)
+        s = run_lbm(fsi.LBMSolver3D, cfg, 10, solid=solid)
+        f = s.distribution_numpy()
+        assert np.array_equal(f[solid == 1], np.broadcast_to(np.float32(W * cfg.rho0), f[solid == 1].shape))
+        u = s.velocity_numpy()
+        assert np.all(u[solid == 1] == 0.0)
+        assert np.all(s.density_numpy()[solid == 1] == np.float32(cfg.rho0))
 
 
 def test_solid_mask_and_neighbour_bits_are_exact():

@@ -118,35 +118,40 @@ def test_polar_rotation_properties():
             assert np.all(np.linalg.eigvalsh(S) > 0)
 
 
-def test_mrt_tables_of_the_kernels():
-    """The exact tables compiled into the kernels: M^-1 M = I, the Guo moment
-    closed form equals the reference's sum over q, and M^-1 rounds to the
-    reference's f32(inv_f64(M)) up to its ~1e-17 noise entries."""
-    import re
-    from fractions import Fraction
-    from pathlib import Path
-
-    text = (Path(__file__).resolve().parents[1] / "taichi-mpm-lbm_amd" / "csrc" / "mrt_tables.h").read_text()
-
-    def table(name):
-        body = re.search(name + r"[^=]*= \{(.*?)\};", text, re.S).group(1)
-        return [int(v) for v in re.findall(r"-?\d+", body)]
-
-    inv = np.array([float(Fraction(a, b)) for a, b in zip(table("INV_NUM"), table("INV_DEN"))]).reshape(19, 19)
+def test_two_relaxation_time_form_of_the_collision():
+    """The lattice kernels collide in the two-relaxation-time form
+    f* = F - s_v (F - g)^+ - s_o (F - g)^- + w e.F + c_v w ((e.u)(e.F) - u.F)
+    - (1 - c_v) (2/3) w u.F, g = w (rho_F + 3 e.j + 4.5 (e.u)^2 - 1.5 u.u).
+    It must equal the reference's matrix form m = M F; m -= S (m - meq);
+    m += (1 - S/2) G; f = M^-1 m (fsi/lbm3d.py:270-333) for any F, u, force
+    (the collision's rho and u need not be F's own), at both rates."""
     M = po.MRT_MATRIX
-    assert np.allclose(inv @ M, np.eye(19), atol=1e-14)
-    ref32 = np.linalg.inv(M).astype(np.float32)
-    assert np.abs(inv.astype(np.float32) - ref32).max() < 1e-7
-
-    ga = np.array([float(Fraction(a, b)) for a, b in zip(table("GA_NUM"), table("GA_DEN"))]).reshape(19, 3)
-    gb = np.array([float(Fraction(a, b)) for a, b in zip(table("GB_NUM"), table("GB_DEN"))])
-    gc = np.array([float(Fraction(a, b)) for a, b in zip(table("GC_NUM"), table("GC_DEN"))]).reshape(19, 3, 3)
-    rng = np.random.default_rng(1)
+    Minv = np.linalg.inv(M)
+    E = po.E.astype(np.float64)
     w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
-    for _ in range(10):
-        u = rng.standard_normal(3) * 0.1
-        f = rng.standard_normal(3) * 1e-3
-        direct = np.array([sum(w[q] * ((po.E[q] - u) @ f + (po.E[q] @ u) * (po.E[q] @ f)) * M[k, q]
-                               for q in range(19)) for k in range(19)])
-        closed = ga @ f - gb * (u @ f) + np.einsum("kab,a,b->k", gc, u, f)
-        assert np.allclose(closed, direct, atol=1e-15)
+    rng = np.random.default_rng(5)
+
+    def meq(rho, u):
+        ux, uy, uz = u
+        m = np.zeros(19)
+        m[0], m[1], m[3], m[5], m[7] = rho, u @ u, ux, uy, uz
+        m[9], m[11] = 2 * ux * ux - uy * uy - uz * uz, uy * uy - uz * uz
+        m[13], m[14], m[15] = ux * uy, uy * uz, ux * uz
+        return m
+
+    for tau in (0.51, 0.55, 0.8, 1.7):
+        S = po.relaxation_rates((tau - 0.5) / 3.0)
+        s_v, s_o = S[1], S[4]
+        c_v = 1 - s_v / 2
+        for _ in range(20):
+            F = w * (1 + rng.normal(0, 0.05, 19))
+            rho, u, fo = rng.uniform(0.9, 1.1), rng.normal(0, 0.05, 3), rng.normal(0, 1e-3, 3)
+            G = M @ (w * ((E - u) @ fo + (E @ u) * (E @ fo)))
+            m = M @ F
+            ref = Minv @ (m - S * (m - meq(rho, u)) + (1 - 0.5 * S) * G)
+            g = w * (F.sum() + 3 * E @ (E.T @ F) + 4.5 * (E @ u) ** 2 - 1.5 * u @ u)
+            h = F - g
+            hp, hm = 0.5 * (h + h[po.LR]), 0.5 * (h - h[po.LR])
+            trt = (F - s_v * hp - s_o * hm + w * (E @ fo) + c_v * w * ((E @ u) * (E @ fo) - u @ fo)
+                   - (1 - c_v) * (2 / 3) * w * (u @ fo))
+            assert np.abs(trt - ref).max() < 1e-15
