@@ -16,6 +16,10 @@
  *    message for the calling thread. Nothing throws across the ABI.
  *  - fields are SoA: component c of cell n lives at [c * ncells + n]; the cell
  *    index is n = (i * ny + j) * nz + k, the reference's row-major order.
+ *    Population buffers (19 * ncells entries of the storage type) are blocked
+ *    by lattice row instead: population q of cell n lives at
+ *    [((i * ny + j) * 19 + q) * nz + k]; fsi_lbm_export_populations returns
+ *    them in the reference's (n, 19) order.
  *  - `real` buffers are float for FSI_REAL_F32 and double for FSI_REAL_F64.
  */
 #ifndef FSI_HIP_H_

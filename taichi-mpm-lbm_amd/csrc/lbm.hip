@@ -1,9 +1,15 @@
 // D3Q19 MRT lattice Boltzmann kernels for gfx950.
 //
-// State layout in HBM (SoA, one plane per population):
-//   pop[q * N + n], n = (i*ny + j)*nz + k   (k fastest, as the reference)
-// so a wavefront of 64 consecutive cells touches 64 consecutive entries of
-// every population plane.
+// State layout in HBM: populations blocked by lattice row. Cell n =
+// (i*ny + j)*nz + k (k fastest, as the reference) keeps population q at
+//   pop[((i*ny + j) * 19 + q) * nz + k]
+// i.e. each z-row holds its 19 population runs of nz values back to back, so
+// a wavefront of 64 consecutive cells touches 64 consecutive entries of each
+// run, and a workgroup's stores fill one contiguous 19*nz block per row. On
+// the pull pattern this layout moves its bytes 3% (FP32) to 9% (16-bit)
+// faster than 19 lattice-sized planes: the workgroups in flight stream from
+// ~9 row blocks instead of 19 planes far apart. The macroscopic fields, the
+// force and the neighbour mask stay plain SoA planes indexed by n.
 //
 // Pull scheme. Between steps the lattice is in one of two states:
 //   post: pop holds post-collision populations f* of the last step;
@@ -324,6 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_lbm_init(LbmK<typename PopStore<S>::
   const int j = (int)(ij % c.ny);
   const int i = (int)(ij / c.ny);
   const Nbr o = neighbours(i, j, k, c.nx, c.ny, c.nz);
+  const int64_t rb = ij * 19 * c.nz + k;  // population q at rb + q * nz
   uint32_t m = 0;
 #pragma unroll
   for (int q = 0; q < 19; ++q)
@@ -346,8 +353,8 @@ __global__ __launch_bounds__(kBlock) void k_lbm_init(LbmK<typename PopStore<S>::
   for (int q = 0; q < 19; ++q) {
     const R v = feq_dev<R, PopStore<S>::shifted>(q, c.rho0, c.rho0, u);
     const S s = PopStore<S>::store(PopStore<S>::scaled(v, fac.st[wclass(q)]));
-    pa[q * c.N + n] = s;
-    pb[q * c.N + n] = s;
+    pa[rb + q * c.nz] = s;
+    pb[rb + q * c.nz] = s;
   }
 }
 
@@ -414,8 +421,6 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   const bool write = live && (!ADAPT || c.pass != 1);
   const uint32_t n = live ? n_lane : N - 1;
   constexpr uint32_t B = sizeof(S);
-  const uint32_t plane = N * B;  // byte size of one population plane
-  const uint32_t own = n * B;
   const uint32_t mask = nbmask[n];
   const bool solid = (mask & 1u) != 0;
 
@@ -424,6 +429,9 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   const uint32_t ij = n / nz;
   const uint32_t j = ij % ny;
   const uint32_t i = ij / ny;
+  const uint32_t run = nz * B;                // byte size of one population run
+  const uint32_t rstride = 19 * nz;           // elements per row block
+  const uint32_t own = (ij * rstride + k) * B;  // population q at own + q * run
 
   // 16-bit storages: all 19 loads are issued before any is used (a barrier
   // to the instruction scheduler: interleaving each conversion with its load
@@ -432,7 +440,7 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   S raw[19];
   if constexpr (PHASE == PHASE_COLLIDE) {
 #pragma unroll
-    for (int q = 0; q < 19; ++q) raw[q] = ld_off(pin, q * plane + own);
+    for (int q = 0; q < 19; ++q) raw[q] = ld_off(pin, q * run + own);
   } else {
     // gather f*(c - e_q, q), or f*(c, opp q) when c - e_q = c + e_opp(q) is
     // solid (fsi/lbm3d.py:352-361 read backwards). One-cell periodic wrap on
@@ -440,25 +448,27 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
     // reads the plain pull sources, which lie in the cache lines its fluid
     // neighbours in the wavefront read anyway (its own slots would be extra
     // lines for the 16 planes shifted in x or y).
-    const uint32_t sx = ny * nz;
+    // element offsets of the neighbouring row blocks and z entries (modulo 2^32)
+    const uint32_t sx = ny * rstride, sy = rstride;
     const uint32_t oxm = i == 0 ? (nx - 1) * sx : (uint32_t)(-(int32_t)sx);
     const uint32_t oxp = i == nx - 1 ? (uint32_t)(-(int32_t)((nx - 1) * sx)) : sx;
-    const uint32_t oym = j == 0 ? (ny - 1) * nz : (uint32_t)(-(int32_t)nz);
-    const uint32_t oyp = j == ny - 1 ? (uint32_t)(-(int32_t)((ny - 1) * nz)) : nz;
+    const uint32_t oym = j == 0 ? (ny - 1) * sy : (uint32_t)(-(int32_t)sy);
+    const uint32_t oyp = j == ny - 1 ? (uint32_t)(-(int32_t)((ny - 1) * sy)) : sy;
     const uint32_t ozm = k == 0 ? nz - 1 : (uint32_t)(-1);
     const uint32_t ozp = k == nz - 1 ? (uint32_t)(-(int32_t)(nz - 1)) : 1u;
     const uint32_t all = solid ? 0u : mask;
+    const uint32_t ownel = ij * rstride + k;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
-      uint32_t src = n;  // c - e_q
+      uint32_t src = ownel;  // c - e_q
       if (ex(q) > 0) src += oxm;
       if (ex(q) < 0) src += oxp;
       if (ey(q) > 0) src += oym;
       if (ey(q) < 0) src += oyp;
       if (ez(q) > 0) src += ozm;
       if (ez(q) < 0) src += ozp;
-      const uint32_t a0 = q * plane + src * B;
-      const uint32_t a1 = opp(q) * plane + own;
+      const uint32_t a0 = q * run + src * B;
+      const uint32_t a1 = opp(q) * run + own;
       const uint32_t sel = 0u - ((all >> opp(q)) & 1u);
       raw[q] = ld_off(pin, a0 ^ ((a0 ^ a1) & sel));
     }
@@ -563,7 +573,7 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   }
   if (write) {
 #pragma unroll
-    for (int q = 0; q < 19; ++q) st_stream(pout, q * plane + own, PopStore<S>::store(x[q]));
+    for (int q = 0; q < 19; ++q) st_stream(pout, q * run + own, PopStore<S>::store(x[q]));
   }
   if constexpr (ADAPT) {
     if (c.pass == 1 || i16_tracked(c.epoch)) {  // uniform
@@ -618,6 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_export_pop(LbmK<typename PopStore<S>
   using R = typename PopStore<S>::real;
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= c.N) return;
+  const int64_t rb = (n / c.nz) * 19 * c.nz + n % c.nz;
   R ld[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -631,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void k_export_pop(LbmK<typename PopStore<S>
   }
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const R dq = PopStore<S>::load(pop[q * c.N + n], ld[wclass(q)]);
+    const R dq = PopStore<S>::load(pop[rb + q * c.nz], ld[wclass(q)]);
     out[n * 19 + q] = PopStore<S>::shifted ? wq<R>(q) * c.rho0 + dq : dq;
   }
 }

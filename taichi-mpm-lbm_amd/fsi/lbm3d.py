@@ -7,8 +7,9 @@ tensors; every update is a HIP kernel of ``libfsi_hip.so``.
 Between steps the lattice is in one of two states:
 
 * materialised - ``pop[cur]`` holds the reference's ``f`` (post-stream,
-  post-boundary populations, SoA (19, N)) and ``rho``/``vel`` hold its
-  macroscopic fields. ``initialize`` leaves this state.
+  post-boundary populations, 19 runs of nz values per lattice row) and
+  ``rho``/``vel`` hold its macroscopic fields. ``initialize`` leaves this
+  state.
 * post-collision - ``pop[cur]`` holds the populations right after the last
   collision; streaming is deferred to the next kernel. ``step`` leaves this
   state.
@@ -250,7 +251,8 @@ class LBMSolver3D:
 
     @property
     def populations(self) -> torch.Tensor:
-        """Current population buffer (SoA, storage dtype)."""
+        """Current population buffer (storage dtype; population q of cell
+        (i, j, k) at ((i*ny + j)*19 + q)*nz + k)."""
         return self.pop[self._cur]
 
     def _force_arg(self) -> nat.LbmForce:
