@@ -425,13 +425,10 @@ constexpr int kZeroBoxMaxNodes = 8192;
 
 // Per-workgroup P2G accumulation tile in LDS: 8 values per node (grid
 // momentum xyz + mass, coupling force density xyz + phi), value-major with a
-// fixed stride of kNodes words, so the value index is an immediate offset of
-// the LDS atomic. The words are FP64 for both precisions: on gfx950
-// ds_add_f64 issues at the integer-atomic rate, while ds_add_f32 is processed
-// lane by lane (measured ~20x slower per wave instruction), and the FP64
-// partial sums are rounded to FP32 once per workgroup at the flush.
+// fixed stride of kNodes 8-byte words, so the value index is an immediate
+// offset of the LDS atomic. The word type is TileAcc<R>::word.
 struct Tile {
-  using acc = double;
+  static constexpr int kWordBytes = 8;
   static constexpr int kValues = 8;
   // a work-list chunk (two z-adjacent 4x4x4 cell blocks) spans 6x6x10 = 360
   // nodes plus drift since the last sort; 496 nodes keep the tile (31 KB)
@@ -451,8 +448,46 @@ template <typename R> constexpr int g2p_lds_bytes() { return kG2PNodes * 3 * (in
 // reaches after its last read of the G2P stage. Sharing takes the fused FP32
 // coupling kernel from 46 KB to 39.9 KB per workgroup, so four fit a CU.
 template <typename R, bool COUPLE> constexpr int p2g_lds_bytes() {
-  return Tile::kNodes * Tile::kValues * (int)sizeof(Tile::acc) + (COUPLE ? Tile::kNodes : 1) * 4 * (int)sizeof(R);
+  return Tile::kNodes * Tile::kValues * Tile::kWordBytes + (COUPLE ? Tile::kNodes : 1) * 4 * (int)sizeof(R);
 }
+
+// Tile words. FP64 build: FP64 sums (ds_add_f64). FP32 build: 64-bit fixed
+// point (ds_add_u64). On gfx950 an LDS atomic wave instruction on scattered
+// addresses costs ~28 clocks as ds_add_f64, ~16 as ds_add_u64 and ~196 as
+// ds_add_f32 (processed lane by lane), and the tile atomics are most of the
+// fused substep. Each workgroup scales its FP32 contributions by a power of
+// two 2^S chosen from the largest magnitude any of its lanes adds (TileScale),
+// so every node sum of its <= 256 particles fits a signed 64-bit integer; a
+// contribution keeps its FP32 product's precision down to 2^-S ~ 2^-54 of
+// that magnitude, and the sums are exact (independent of the order of the
+// atomics). The flush converts each sum back once per node and value.
+template <typename R> struct TileAcc;
+template <> struct TileAcc<double> {
+  using word = double;
+  using P = double;  // type of the per-node products
+  static constexpr bool fixed = false;
+  __device__ static void add(word* a, double c) { atomicAdd(a, c); }
+  __device__ static double value(word s, double) { return s; }
+};
+template <> struct TileAcc<float> {
+  using word = unsigned long long;
+  using P = float;
+  static constexpr bool fixed = true;
+  // c: the contribution times 2^(S-32), |c| < 2^22. Two's complement
+  // floor(c) * 2^32 + fract(c) * 2^32 (the fraction truncated to 32 bits).
+  __device__ static void add(word* a, float c) {
+    int32_t hi;  // floor(c), one instruction (the compiler emits floor + convert)
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(c));
+    const uint32_t lo = (uint32_t)(__builtin_amdgcn_fractf(c) * 4294967296.0f);
+    atomicAdd(a, ((word)(uint32_t)hi << 32) | (word)lo);
+  }
+  // sum * 2^-S (inv = 2^-S; NaN when the workgroup added a non-finite value)
+  __device__ static float value(word s, double inv) {
+    return (float)((double)(long long)s * inv);
+  }
+};
+
+
 template <typename R, bool COUPLE> constexpr int pass_lds_bytes() {
   return p2g_lds_bytes<R, COUPLE>() > g2p_lds_bytes<R>() ? p2g_lds_bytes<R, COUPLE>() : g2p_lds_bytes<R>();
 }
@@ -472,6 +507,42 @@ __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
   return v;
+}
+
+// Power-of-two scale of one batch of a workgroup's tile contributions (all
+// contributions of the batch have |c| <= the workgroup maximum of `bound`):
+// mul = 2^(S-32) applied to each FP32 contribution before TileAcc::add, inv =
+// 2^-S for the flush. Every thread calls it (it synchronises the workgroup).
+struct TileScale {
+  float mul;
+  double inv;
+};
+template <int NW>
+__device__ __forceinline__ TileScale tile_scale(float bound, uint32_t* red) {
+  // nonnegative floats order as their bit patterns; NaN (0x7fc00000) and
+  // inf order above every finite value
+  const float b = bound >= 0.0f ? bound : (bound == bound ? -bound : __int_as_float(0x7fc00000));
+  const uint32_t m = (uint32_t)wave_max((int)__float_as_uint(b));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  uint32_t w = red[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) w = max(w, red[i]);
+  const float mx = __uint_as_float(w);
+  TileScale s;
+  if (!(mx < __builtin_huge_valf())) {  // inf or NaN: sums become NaN at the flush
+    s.mul = 0.0f;
+    s.inv = __builtin_nan("");
+    return s;
+  }
+  // mx < 2^e; 256 contributions per node sum below 2^(e+8); S = 54 - e keeps
+  // the sums below 2^62 and each scaled contribution below 2^22
+  int e = 0;
+  (void)frexpf(mx, &e);
+  e = min(max(e, -100), 100);
+  s.mul = ldexpf(1.0f, 22 - e);
+  s.inv = ldexp(1.0, e - 54);
+  return s;
 }
 
 // ----------------------------------------------- workgroup node box ----
@@ -555,7 +626,10 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
                                          int32_t* __restrict__ boxes_out, int32_t* __restrict__ big_flag,
                                          int big_value, const R* __restrict__ closure, const R* rx, const R* rv,
                                          const R* rC) {
-  using A = Tile::acc;
+  using TA = TileAcc<R>;
+  using A = typename TA::word;
+  using P = typename TA::P;
+  static_assert(sizeof(A) == Tile::kWordBytes, "tile word size");
   constexpr int kN = Tile::kNodes;
   A* tile = reinterpret_cast<A*>(pass_lds<pass_lds_bytes<R, COUPLE>()>());
   // coupling: lattice velocity and a fluid flag per node of the box (zero
@@ -564,6 +638,9 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
                                            kN * Tile::kValues * sizeof(A));
   constexpr int kS = Slots<PL>::value;
   __shared__ int red[6][PL * kS / 64];
+  // fixed-point tile: maxima of the coupling and the P2G contributions
+  __shared__ uint32_t sred[2][PL * kS / 64];
+  TileScale cscale{1.0f, 1.0}, pscale{1.0f, 1.0};
   // plane split (PL = 3): partial support and interpolated velocity
   __shared__ R cpart[PL > 1 && COUPLE ? PL : 1][PL > 1 && COUPLE ? kS : 1][4];
   static_assert(PL == 1 || (PL == 3 && sizeof(R) == 4), "plane split: FP32 only");
@@ -694,17 +771,17 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     row[s] = b0 + jo[s] * tzi;
     plane[s] = io[s] * tyz;
   }
-  A wir[3], wjr[3], wkr[3];
-  rotate3<A>(ri, w[0][0], w[1][0], w[2][0], wir);
-  rotate3<A>(rj, w[0][1], w[1][1], w[2][1], wjr);
-  rotate3<A>(rk, w[0][2], w[1][2], w[2][2], wkr);
+  P wir[3], wjr[3], wkr[3];
+  rotate3<P>(ri, w[0][0], w[1][0], w[2][0], wir);
+  rotate3<P>(rj, w[0][1], w[1][1], w[2][1], wjr);
+  rotate3<P>(rk, w[0][2], w[1][2], w[2][2], wkr);
   // plane split: this wavefront's x plane of the stencil is the first (and
   // only) one of the static order
   constexpr int kPlanes = PL > 1 ? 1 : 3;
   if constexpr (PL > 1) {
     io[0] = pln;
     plane[0] = pln * tyz;
-    wir[0] = A(pick3(pln, w[0][0], w[1][0], w[2][0]));
+    wir[0] = P(pick3(pln, w[0][0], w[1][0], w[2][0]));
   }
 
   if constexpr (COUPLE) {
@@ -763,6 +840,9 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
         for (int a = 0; a < 3; ++a) u[a] = (cpart[0][slot][1 + a] + cpart[1][slot][1 + a]) + cpart[2][slot][1 + a];
       }
     }
+    // the reaction values this lane spreads on the tile (scaled by 1/wsum)
+    bool spreads = false;
+    P rfn[4] = {P(0), P(0), P(0), P(0)};
     if (act) {
       // global path (box too large for the tile): the nodes are gathered in
       // batches of NB i-planes, all loads of a batch in flight together;
@@ -845,44 +925,13 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
         const R phi = vp_vol / cp.cell_volume;
         const R rf[3] = {-pf[0] * fscale, -pf[1] * fscale, -pf[2] * fscale};
         if (use_lds) {
-          // all 27 nodes in the lane-rotated order; the values of invalid
-          // (solid or out-of-lattice) nodes are dropped at the flush, which
-          // leaves the valid nodes' sums unchanged
-          const A iw = A(inv_wsum);
-          const A rfn[4] = {A(rf[0]) * iw, A(rf[1]) * iw, A(rf[2]) * iw, A(phi) * iw};
-          // the value order is rotated by lane as well (instruction k adds
-          // value (k + r) % NV), so lanes that meet on one node in the same
-          // instruction mostly write different values, i.e. different words
-          auto spread = [&](auto with_phi) {
-            constexpr int NV = decltype(with_phi)::value ? 4 : 3;
-            const int r = NV == 4 ? (lane & 3) : (lane >> 2) % 3;
-            A val[NV];
-            int off[NV];
-#pragma unroll
-            for (int k = 0; k < NV; ++k) {
-              const int c = (k + r) % NV;
-              val[k] = NV == 4 ? pick4(c, rfn[0], rfn[1], rfn[2], rfn[3]) : pick3(c, rfn[0], rfn[1], rfn[2]);
-              off[k] = (4 + c) * kN;
-            }
-#pragma unroll
-            for (int ii = 0; ii < kPlanes; ++ii)
-#pragma unroll
-              for (int jj = 0; jj < 3; ++jj) {
-                const A wij = wir[ii] * wjr[jj];
-                A* tr = tile + row[jj] + plane[ii];
-#pragma unroll
-                for (int kk = 0; kk < 3; ++kk) {
-                  const A wt = wij * wkr[kk];
-                  A* tn = tr + ko[kk];
-#pragma unroll
-                  for (int k = 0; k < NV; ++k) atomicAdd(tn + off[k], wt * val[k]);
-                }
-              }
-          };
-          if (cp.build)
-            spread(std::true_type{});
-          else
-            spread(std::false_type{});
+          // spread below, after the workgroup's scale
+          spreads = true;
+          const P iw = P(inv_wsum);
+          rfn[0] = P(rf[0]) * iw;
+          rfn[1] = P(rf[1]) * iw;
+          rfn[2] = P(rf[2]) * iw;
+          rfn[3] = P(phi) * iw;
         } else if (writer) {
 #pragma unroll
           for (int i = 0; i < 3; ++i)
@@ -903,6 +952,54 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
       } else {
         unsupported = true;
         if (writer) cmask[p] = 0;
+      }
+    }
+    if (use_lds) {  // uniform over the workgroup
+      if constexpr (TA::fixed) {
+        const float bnd = spreads ? fmaxf(fmaxf(fabsf((float)rfn[0]), fabsf((float)rfn[1])),
+                                          fmaxf(fabsf((float)rfn[2]), cp.build ? fabsf((float)rfn[3]) : 0.0f))
+                                  : 0.0f;
+        cscale = tile_scale<PL * kS / 64>(bnd, sred[0]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) rfn[k] *= cscale.mul;
+      }
+      if (spreads) {
+        // all 27 nodes in the lane-rotated order; the values of invalid
+        // (solid or out-of-lattice) nodes are dropped at the flush, which
+        // leaves the valid nodes' sums unchanged. The value order is
+        // rotated by lane as well (instruction k adds value (k + r) % NV),
+        // so lanes that meet on one node in the same instruction mostly
+        // write different values, i.e. different words
+        auto spread = [&](auto with_phi) {
+          constexpr int NV = decltype(with_phi)::value ? 4 : 3;
+          const int r = NV == 4 ? (lane & 3) : (lane >> 2) % 3;
+          P val[NV];
+          int off[NV];
+#pragma unroll
+          for (int k = 0; k < NV; ++k) {
+            const int c = (k + r) % NV;
+            val[k] = NV == 4 ? pick4(c, rfn[0], rfn[1], rfn[2], rfn[3]) : pick3(c, rfn[0], rfn[1], rfn[2]);
+            off[k] = (4 + c) * kN;
+          }
+#pragma unroll
+          for (int ii = 0; ii < kPlanes; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 3; ++jj) {
+              const P wij = wir[ii] * wjr[jj];
+              A* tr = tile + row[jj] + plane[ii];
+#pragma unroll
+              for (int kk = 0; kk < 3; ++kk) {
+                const P wt = wij * wkr[kk];
+                A* tn = tr + ko[kk];
+#pragma unroll
+                for (int k = 0; k < NV; ++k) TA::add(tn + off[k], wt * val[k]);
+              }
+            }
+        };
+        if (cp.build)
+          spread(std::true_type{});
+        else
+          spread(std::false_type{});
       }
     }
     if (writer) {
@@ -936,6 +1033,10 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   // plane split: every plane reads F before plane 0 stores the update, so
   // the store waits for the flush barrier
   R Fn[9];
+  // P2G values of this lane for the tile: momentum at the particle and the
+  // affine matrix (scattered after the workgroup's scale)
+  bool scatters = false;
+  R aff[9], mom0[3];
   if constexpr (P2G) {
     if (act) {
 #pragma unroll
@@ -956,57 +1057,82 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
         have_rot = polar_newton(Fn, Rot);
       deferred = !have_rot;
       if (have_rot) {
-        R aff[9], mom0[3];
         affine_momentum(m, Fn, Rot, Cm, vp, pf, mp, vp_vol, aff, mom0);
-        if (use_lds) {
-          // lane-rotated order as for the coupling scatter; in FP64, the
-          // accumulator type, the affine momentum is built up separably:
-          // mom0 + aff (dp_i, dp_j, dp_k) per i plane, (i, j) row and node
-          // The four values (momentum xyz, mass) are written in a
-          // lane-rotated order too: instruction k adds value (k + lane) % 4,
-          // with the mass as a fourth component of zero affine part
-          const A dx = A(m.dx);
-          A dpj[3], dpk[3];
+        if (use_lds)
+          scatters = true;
+        else if (writer)
+          scatter_global(m, grid, base, fx, w, mom0, aff, mp);
+      }
+    }
+    if (use_lds) {  // uniform over the workgroup
+      float mul = 1.0f;
+      if constexpr (TA::fixed) {
+        // largest |mom0 + aff dp| over the stencil: |dp| <= 1.5 dx per axis
+        float bnd = 0.0f;
+        if (scatters) {
+          bnd = fabsf((float)mp);
 #pragma unroll
-          for (int s = 0; s < 3; ++s) {
-            dpj[s] = (A(jo[s]) - A(fx[1])) * dx;
-            dpk[s] = (A(ko[s]) - A(fx[2])) * dx;
+          for (int a = 0; a < 3; ++a)
+            bnd = fmaxf(bnd, fabsf((float)mom0[a]) +
+                                 1.5f * (float)m.dx *
+                                     (fabsf((float)aff[a * 3]) + fabsf((float)aff[a * 3 + 1]) +
+                                      fabsf((float)aff[a * 3 + 2])));
+        }
+        pscale = tile_scale<PL * kS / 64>(bnd, sred[1]);
+        mul = pscale.mul;
+      }
+      if (scatters) {
+        // lane-rotated order as for the coupling scatter; the affine
+        // momentum is built up separably: mom0 + aff (dp_i, dp_j, dp_k) per
+        // i plane, (i, j) row and node. The four values (momentum xyz, mass)
+        // are written in a lane-rotated order too: instruction k adds value
+        // (k + lane) % 4, with the mass as a fourth component of zero affine
+        // part
+        const P dx = P(m.dx);
+        P dpj[3], dpk[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          dpj[s] = (P(jo[s]) - P(fx[1])) * dx;
+          dpk[s] = (P(ko[s]) - P(fx[2])) * dx;
+        }
+        P cm[4], cx[4], cy[4], cz[4];
+        int off[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int c = (k + lane) & 3;
+          cm[k] = P(pick4(c, mom0[0], mom0[1], mom0[2], mp));
+          cx[k] = P(pick4(c, aff[0], aff[3], aff[6], R(0)));
+          cy[k] = P(pick4(c, aff[1], aff[4], aff[7], R(0)));
+          cz[k] = P(pick4(c, aff[2], aff[5], aff[8], R(0)));
+          if constexpr (TA::fixed) {
+            cm[k] *= mul;
+            cx[k] *= mul;
+            cy[k] *= mul;
+            cz[k] *= mul;
           }
-          A cm[4], cx[4], cy[4], cz[4];
-          int off[4];
+          off[k] = c * kN;
+        }
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int c = (k + lane) & 3;
-            cm[k] = A(pick4(c, mom0[0], mom0[1], mom0[2], mp));
-            cx[k] = A(pick4(c, aff[0], aff[3], aff[6], R(0)));
-            cy[k] = A(pick4(c, aff[1], aff[4], aff[7], R(0)));
-            cz[k] = A(pick4(c, aff[2], aff[5], aff[8], R(0)));
-            off[k] = c * kN;
-          }
+        for (int ii = 0; ii < kPlanes; ++ii) {
+          const P dpi = (P(io[ii]) - P(fx[0])) * dx;
+          P mi[4];
 #pragma unroll
-          for (int ii = 0; ii < kPlanes; ++ii) {
-            const A dpi = (A(io[ii]) - A(fx[0])) * dx;
-            A mi[4];
+          for (int k = 0; k < 4; ++k) mi[k] = cm[k] + cx[k] * dpi;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) mi[k] = cm[k] + cx[k] * dpi;
+          for (int jj = 0; jj < 3; ++jj) {
+            const P wij = wir[ii] * wjr[jj];
+            P mij[4];
 #pragma unroll
-            for (int jj = 0; jj < 3; ++jj) {
-              const A wij = wir[ii] * wjr[jj];
-              A mij[4];
+            for (int k = 0; k < 4; ++k) mij[k] = mi[k] + cy[k] * dpj[jj];
+            A* tr = tile + row[jj] + plane[ii];
 #pragma unroll
-              for (int k = 0; k < 4; ++k) mij[k] = mi[k] + cy[k] * dpj[jj];
-              A* tr = tile + row[jj] + plane[ii];
+            for (int kk = 0; kk < 3; ++kk) {
+              const P wt = wij * wkr[kk];
+              A* tn = tr + ko[kk];
 #pragma unroll
-              for (int kk = 0; kk < 3; ++kk) {
-                const A wt = wij * wkr[kk];
-                A* tn = tr + ko[kk];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) atomicAdd(tn + off[k], wt * (mij[k] + cz[k] * dpk[kk]));
-              }
+              for (int k = 0; k < 4; ++k) TA::add(tn + off[k], wt * (mij[k] + cz[k] * dpk[kk]));
             }
           }
-        } else if (writer) {
-          scatter_global(m, grid, base, fx, w, mom0, aff, mp);
         }
       }
     }
@@ -1023,15 +1149,15 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
       const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const R val = R(tile[s * kN + t]);
-        if (val != R(0)) atomicAdd(grid + 4 * node + s, val);
+        const A sum = tile[s * kN + t];
+        if (sum != A(0)) atomicAdd(grid + 4 * node + s, R(TA::value(sum, pscale.inv)));
       }
       if constexpr (COUPLE) {
         if (ctile[t][3] != R(0)) {
 #pragma unroll
           for (int s = 4; s < Tile::kValues; ++s) {
-            const R val = R(tile[s * kN + t]);
-            if (val != R(0)) atomicAdd(cforce + 4 * node + (s - 4), val);
+            const A sum = tile[s * kN + t];
+            if (sum != A(0)) atomicAdd(cforce + 4 * node + (s - 4), R(TA::value(sum, cscale.inv)));
           }
         }
       }
