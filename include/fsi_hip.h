@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FSI_ABI_VERSION 2
+#define FSI_ABI_VERSION 3
 
 enum {
   FSI_OK = 0,
@@ -212,6 +212,12 @@ typedef struct fsi_mpm_buffers {
                               counter rows, fsi_mpm_substeps runs G2P of substep s-1
                               and P2G of substep s in one launch (S + 1 launches for
                               S substeps) */
+  const void* material;    /* optional 2*cap real: per-particle Lame parameters, 2 mu at
+                              [p] and lambda at [cap + p], used by the P2G stress instead
+                              of fsi_mpm_params.two_mu / lam. No counterpart in the
+                              reference (one global E and nu, fsi/mpm3d.py:33-38): a
+                              build-only extension for scenes of bodies of different
+                              stiffness. NULL: the global parameters */
 } fsi_mpm_buffers;
 
 /* Coupling inputs/outputs read and written inside the particle passes
@@ -309,7 +315,8 @@ int fsi_mpm_sort_workspace(const fsi_mpm_params* p, int64_t* bytes);
  * truncating base cell of fsi/mpm3d.py:239 (computed as P2G computes it),
  * clamped to [-1, n] per axis, in 4x4x4-blocked order (blocks row-major,
  * an even number of blocks in z); inactive particles last. The permutation is that of a stable sort of the keys (ties keep
- * index order). Every particle array of `src` is gathered into `dst` in
+ * index order). Every particle array of `src` (with the material plane, when
+ * src->material is set; dst->material must then be set too) is gathered into `dst` in
  * sorted order and ids_dst[t] = ids_src[perm[t]]; keys_out / perm_out (n
  * each, optional) receive the sorted keys and the permutation. With
  * `chunks` (2 * max_chunks int32, optional) it also writes the P2G work list

@@ -20,6 +20,7 @@ template <typename R> struct MpmK {
   int cap;
   R dx, inv_dx, dt;
   R two_mu, lam;
+  const R* material;  // optional per-particle (2 mu, lambda) planes, 2 x cap
   R g[3];
   int bw;
   R damping;
@@ -28,8 +29,9 @@ template <typename R> struct MpmK {
   R lower, upper[3];
 };
 
-template <typename R> MpmK<R> make_mpm(const fsi_mpm_params* p) {
+template <typename R> MpmK<R> make_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b = nullptr) {
   MpmK<R> m;
+  m.material = b ? (const R*)b->material : nullptr;
   m.nx = p->nx;
   m.ny = p->ny;
   m.nz = p->nz;
@@ -352,19 +354,27 @@ __device__ __forceinline__ void deformation_update(R dt, const R Cm[9], const R 
 // P = 2 mu (F - R) F^T + lam J (J - 1) I, aff = -dt vol 4/dx^2 P + m C,
 // mom0 = m v + dt f (the coupling force of this substep)
 template <typename R>
-__device__ __forceinline__ void affine_momentum(const MpmK<R>& m, const R Fn[9], const R Rot[9], const R Cm[9],
-                                                const R vp[3], const R pf[3], R mp, R vp_vol, R aff[9],
-                                                R mom0[3]) {
+__device__ __forceinline__ void affine_momentum(const MpmK<R>& m, int p, const R Fn[9], const R Rot[9],
+                                                const R Cm[9], const R vp[3], const R pf[3], R mp, R vp_vol,
+                                                R aff[9], R mom0[3]) {
+  // Lame parameters: the material's (fsi/mpm3d.py:33-38), or the particle's
+  // own when a per-particle material plane is given (no counterpart in the
+  // reference, which has one global E and nu)
+  R two_mu = m.two_mu, lam = m.lam;
+  if (m.material) {
+    two_mu = m.material[p];
+    lam = m.material[m.cap + p];
+  }
   const R J = det3(Fn);
   const R scale = -m.dt * vp_vol * R(4) * m.inv_dx * m.inv_dx;
-  const R lj = m.lam * J * (J - R(1));
+  const R lj = lam * J * (J - R(1));
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int s2 = 0; s2 < 3; ++s2) {
       R st = R(0);
 #pragma unroll
-      for (int t = 0; t < 3; ++t) st += (m.two_mu * (Fn[r * 3 + t] - Rot[r * 3 + t])) * Fn[s2 * 3 + t];
+      for (int t = 0; t < 3; ++t) st += (two_mu * (Fn[r * 3 + t] - Rot[r * 3 + t])) * Fn[s2 * 3 + t];
       if (r == s2) st += lj;
       aff[r * 3 + s2] = scale * st + mp * Cm[r * 3 + s2];
     }
@@ -1057,7 +1067,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
         have_rot = polar_newton(Fn, Rot);
       deferred = !have_rot;
       if (have_rot) {
-        affine_momentum(m, Fn, Rot, Cm, vp, pf, mp, vp_vol, aff, mom0);
+        affine_momentum(m, p, Fn, Rot, Cm, vp, pf, mp, vp_vol, aff, mom0);
         if (use_lds)
           scatters = true;
         else if (writer)
@@ -1198,7 +1208,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
       R Rot[9], aff[9], mom0[3];
 #pragma unroll
       for (int r = 0; r < 9; ++r) Rot[r] = R(Rdd[r]);
-      affine_momentum(m, Fr, Rot, Cr, vr, pr, mr, volr, aff, mom0);
+      affine_momentum(m, p, Fr, Rot, Cr, vr, pr, mr, volr, aff, mom0);
       scatter_global(m, grid, br, fr, wr, mom0, aff, mr);
     }
   }
@@ -1765,7 +1775,7 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
           const fsi_coupling_buffers* cb, int clear_force, int do_p2g, bool zero_counters,
           R* grid_target, int32_t* zero_ctr, hipStream_t st, int32_t* boxes_out = nullptr,
           int32_t* big_flag = nullptr, int big_value = 0, int32_t* counters_override = nullptr) {
-  MpmK<R> m = make_mpm<R>(p);
+  MpmK<R> m = make_mpm<R>(p, b);
   int32_t* ctrs = counters_override ? counters_override : (cb ? cb->counters : nullptr);
   CplK<R> cp = make_cpl<R>(c);
   cp.force_mean = cb ? (R*)cb->force_mean : nullptr;
@@ -1823,7 +1833,7 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
                void* zero_grid, int32_t* zero_ctr, hipStream_t st, const int32_t* boxes_in = nullptr,
                const int32_t* big_flag = nullptr, int big_expect = 0, int num_boxes = 0,
                int32_t* counters_override = nullptr) {
-  MpmK<R> m = make_mpm<R>(p);
+  MpmK<R> m = make_mpm<R>(p, b);
   int32_t* ctrs = counters_override ? counters_override : (cb ? cb->counters : nullptr);
   CplK<R> cp = make_cpl<R>(c);
   const bool contact = c && c->contact_enabled;
@@ -1928,7 +1938,7 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
     return rc;
   local.build_fraction = 0;
   const fsi_coupling_params* lc = c ? &local : nullptr;
-  MpmK<R> m = make_mpm<R>(p);
+  MpmK<R> m = make_mpm<R>(p, b);
   CplK<R> cp = make_cpl<R>(lc);
   cp.force_mean = cb ? (R*)cb->force_mean : nullptr;
   const bool couple = lc && lc->enabled;
@@ -2038,7 +2048,7 @@ int substeps_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
 template <typename R>
 int contact_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
               const fsi_coupling_buffers* cb, hipStream_t st) {
-  MpmK<R> m = make_mpm<R>(p);
+  MpmK<R> m = make_mpm<R>(p, b);
   CplK<R> cp = make_cpl<R>(c);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 3, 2);
   FSI_TRY_LAUNCH("zero_counters");

@@ -317,6 +317,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(int num, int cap, const int32
   }
   ((R*)dst.mass)[t] = ((const R*)src.mass)[s];
   ((R*)dst.volume)[t] = ((const R*)src.volume)[s];
+  if (src.material) {
+    ((R*)dst.material)[t] = ((const R*)src.material)[s];
+    ((R*)dst.material)[c + t] = ((const R*)src.material)[c + s];
+  }
   ((int32_t*)dst.active)[t] = src.active[s];
   ids_dst[t] = ids_src[s];
   if (keys_out) keys_out[t] = keys[s];
@@ -413,6 +417,8 @@ int fsi_mpm_sort(const fsi_mpm_params* p, const fsi_mpm_buffers* src, const fsi_
   FSI_REQUIRE(dst->x && dst->v && dst->C && dst->F && dst->mass && dst->volume && dst->active && dst->pforce,
               "mpm_sort: null destination particle buffer");
   FSI_REQUIRE(src->x != dst->x && ids_src != ids_dst, "mpm_sort: source and destination must differ");
+  FSI_REQUIRE((src->material == nullptr) == (dst->material == nullptr),
+              "mpm_sort: a per-particle material plane needs a destination plane (and only then)");
   FSI_REQUIRE(p->num_particles >= 0 && p->num_particles <= p->capacity, "mpm_sort: bad particle count");
   int64_t need = 0;
   if (int rc = fsi_mpm_sort_workspace(p, &need)) return rc;

@@ -13,6 +13,8 @@ from pathlib import Path
 import torch
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libfsi_hip.so"
+# include/fsi_hip.h FSI_ABI_VERSION the structures below follow
+ABI_VERSION = 3
 _lib = None
 
 POP_F32, POP_F16, POP_F64, POP_I16 = 0, 1, 2, 3
@@ -74,7 +76,8 @@ class MpmParams(ctypes.Structure):
 class MpmBuffers(ctypes.Structure):
     _fields_ = [("x", _vp), ("v", _vp), ("C", _vp), ("F", _vp), ("mass", _vp), ("volume", _vp),
                 ("active", _vp), ("pforce", _vp), ("grid", _vp), ("grid_alt", _vp), ("chunks", _vp),
-                ("zero_boxes", _vp), ("zero_boxes_cap", ctypes.c_int32), ("grid_alt2", _vp)]
+                ("zero_boxes", _vp), ("zero_boxes_cap", ctypes.c_int32), ("grid_alt2", _vp),
+                ("material", _vp)]
 
 
 class CouplingParams(ctypes.Structure):
@@ -163,6 +166,11 @@ def load_library() -> ctypes.CDLL:
         lib.fsi_mlp_train_workspace.restype = ctypes.c_int
         lib.fsi_last_error.argtypes = []
         lib.fsi_last_error.restype = ctypes.c_char_p
+        if lib.fsi_abi_version() != ABI_VERSION:
+            raise RuntimeError(
+                f"{_LIB_PATH} implements ABI version {lib.fsi_abi_version()}, the bindings expect "
+                f"{ABI_VERSION}: rebuild it (`python taichi-mpm-lbm_amd/build.py`)"
+            )
         _lib = lib
     return _lib
 

@@ -93,6 +93,9 @@ class MPMSolver3D:
         # plane of 9 at a time with fewer registers (0: always the latter)
         self.wide_max_workgroups = 512
         self.num_particles = 0
+        # optional per-particle Lame parameters (2 mu, lambda) in slot order,
+        # 2 x cap (set_particle_material); None: the config's E and nu
+        self.material = None
         self._buffers = self._make_buffers()
         self._initialized = False
 
@@ -195,8 +198,41 @@ class MPMSolver3D:
         self._permuted = False
         self._num_chunks = 0
         self.num_particles = count
+        self.material = None
+        self._buffers = self._make_buffers()
         self._initialized = True
         return count
+
+    def set_particle_material(self, youngs_modulus, poisson_ratio=None) -> None:
+        """Per-particle elastic material (no counterpart in the reference,
+        whose MPMConfig has one global E and nu, fsi/mpm3d.py:33-38): Young's
+        modulus and Poisson ratio per particle in input order (scalars
+        broadcast; poisson_ratio defaults to the config's). The P2G stress
+        then uses each particle's own Lame parameters, computed as the
+        reference computes the global ones. initialize_particles_* clears it."""
+        if not self._initialized:
+            raise RuntimeError("MPMSolver3D has no particles to give a material.")
+        n = self.num_particles
+        E = np.broadcast_to(np.asarray(youngs_modulus, dtype=np.float64), (n,))
+        nu = np.broadcast_to(np.asarray(self.config.poisson_ratio if poisson_ratio is None else poisson_ratio,
+                                        dtype=np.float64), (n,))
+        if not (np.all(np.isfinite(E)) and np.all(E > 0.0)):
+            raise ValueError("youngs_modulus must be positive and finite.")
+        if not (np.all(nu > -1.0) and np.all(nu < 0.5)):
+            raise ValueError("poisson_ratio must lie in (-1, 0.5).")
+        mu = E / (2.0 * (1.0 + nu))
+        lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+        cap = self.max_particles
+        planes = np.zeros((2, cap), self._np_real)
+        order = self.ids[:n].cpu().numpy() if self._permuted else np.arange(n)
+        planes[0, :n] = 2.0 * mu[order]
+        planes[1, :n] = lam[order]
+        if self.material is None:
+            self.material = torch.empty(2 * cap, dtype=self.real_dtype, device=self.device)
+        self.material.copy_(torch.from_numpy(planes.reshape(-1)))
+        if self._sort is not None and "material" not in self._sort["alt"]:
+            self._sort["alt"]["material"] = torch.empty_like(self.material)
+        self._buffers = self._make_buffers()
 
     def _default_spacing(self) -> np.ndarray:
         return np.full(3, self.dx / float(self.config.particles_per_cell), dtype=np.float32)
@@ -279,6 +315,8 @@ class MPMSolver3D:
                 "particle_force": torch.empty(3 * cap, dtype=rd, device=dev),
                 "ids": torch.empty(cap, dtype=torch.int32, device=dev),
             }
+            if self.material is not None:
+                alt["material"] = torch.empty_like(self.material)
             nbytes = ctypes.c_int64(0)
             p = self.params(self.dt_default)
             rc = nat.load_library().fsi_mpm_sort_workspace(nat.ref(p), ctypes.byref(nbytes))
@@ -300,7 +338,7 @@ class MPMSolver3D:
             nat.ptr(self.x), nat.ptr(self.v), nat.ptr(self.C), nat.ptr(self.F),
             nat.ptr(self.mass), nat.ptr(self.volume), nat.ptr(self.active),
             nat.ptr(self.particle_force), nat.ptr(self.grid), nat.ptr(self.grid_alt), nat.ptr(chunks),
-            nat.ptr(self._zero_boxes), self._zero_boxes_cap, nat.ptr(self.grid_alt2),
+            nat.ptr(self._zero_boxes), self._zero_boxes_cap, nat.ptr(self.grid_alt2), nat.ptr(self.material),
         )
 
     def sort_particles(self, keys_out: torch.Tensor | None = None,
@@ -318,11 +356,13 @@ class MPMSolver3D:
             nat.ptr(alt["mass"]), nat.ptr(alt["volume"]), nat.ptr(alt["active"]),
             nat.ptr(alt["particle_force"]), nat.ptr(self.grid), nat.ptr(self.grid_alt),
         )
+        if self.material is not None:
+            dst.material = nat.ptr(alt["material"])
         p = self.params(self.dt_default)
         nat.call("fsi_mpm_sort", nat.ref(p), nat.ref(self._buffers), nat.ref(dst), nat.ptr(self.ids),
                  nat.ptr(alt["ids"]), nat.ptr(st["ws"]), st["ws"].numel(), nat.ptr(keys_out),
                  nat.ptr(perm_out), nat.ptr(st["chunks"]), st["max_chunks"], nat.ptr(st["count"]))
-        for name in self._SORTED_FIELDS + ("ids",):
+        for name in self._SORTED_FIELDS + ("ids",) + (("material",) if self.material is not None else ()):
             cur = getattr(self, name)
             setattr(self, name, alt[name])
             alt[name] = cur

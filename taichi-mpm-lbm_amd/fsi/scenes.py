@@ -138,3 +138,12 @@ def sphere_pack(shape, count, rmin, rmax, spacing=0.5, seed=0, margin=3.0, gap=1
     info = {"centres": np.array(centres), "radii": np.array(radii), "counts": [len(p) for p in parts]}
     return pos, info
 
+
+def body_moduli(counts, low, high, seed=0):
+    """Young's modulus per body, log-uniform in [low, high] (numpy
+    default_rng(seed)), repeated over each body's `counts` particles.
+    Returns (per-particle float64 array, per-body array), for
+    MPMSolver3D.set_particle_material with sphere_pack's info["counts"]."""
+    rng = np.random.default_rng(seed)
+    per_body = np.exp(rng.uniform(np.log(low), np.log(high), len(counts)))
+    return np.repeat(per_body, np.asarray(counts, dtype=np.int64)), per_body
