@@ -180,12 +180,13 @@ def _cpu_task(task, budget_s):
         dt = time.perf_counter() - t0
         out.update(value=done / dt, unit="coupled steps/s", steps=done, seconds=dt)
     elif task == "coupled_large":
-        # the suspension's coupled step: one lattice step plus 112 MPM
+        # the suspension's coupled step: one lattice step plus 256 MPM
         # substeps with the coupling; timed as one lattice step and two
-        # coupled substeps, extrapolated to 112 substeps
+        # coupled substeps, extrapolated to 256 substeps (the CPU oracle has
+        # the reference's one global material; the cost does not depend on E)
         import dataclasses
 
-        cfg, solid, pos = coupled_scene_large()
+        cfg, solid, pos, _ = coupled_scene_large()
         sub = cfg.coupling.mpm_substeps_per_lbm_step
         cfg2 = dataclasses.replace(cfg, coupling=dataclasses.replace(cfg.coupling, mpm_substeps_per_lbm_step=2))
         sim = po.OracleSimulation(cfg2, "f32")
@@ -275,18 +276,26 @@ def coupled_scene_32(storage="fp32"):
     return cfg, solid, pos
 
 
+LARGE_SUBSTEPS = 256
+
+
 def coupled_scene_large(storage="fp32"):
     """256x128x128 coupled suspension: inflow u_x=0.04 (velocity face x-,
     fluid starting at that velocity), pressure face x+, no-slip walls as a
-    solid shell in y/z, tau=0.56;
-    16 non-overlapping elastic spheres r U[6,12] (sphere_pack seed 0),
-    8 particles per cell (spacing 0.5, jitter U(-0.125,0.125)), E=1e3,
-    nu=0.3, density 1.2, gamma=0.5, 112 substeps per lattice step."""
+    solid shell in y/z, tau=0.56; 16 non-overlapping elastic spheres r U[6,12]
+    (sphere_pack seed 0), 8 particles per cell (spacing 0.5, jitter
+    U(-0.125,0.125)), E log-uniform in [5e2, 5e3] per sphere (seed 0, a
+    per-particle material plane), nu=0.3, density 1.2, gamma=0.5.
+    Substeps: the stiffest sphere's P-wave speed sqrt((lam + 2 mu)/rho) is
+    74.9 cells per time unit, and mpm_dt <= 0.3 dx / c_p needs >= 250
+    substeps per lattice step (256 here).
+    Returns (config, solid mask, positions, per-particle E)."""
     from fsi.config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
-    from fsi.scenes import channel_walls, sphere_pack
+    from fsi.scenes import body_moduli, channel_walls, sphere_pack
 
     shape = (256, 128, 128)
     pos, info = sphere_pack(shape, 16, 6.0, 12.0, spacing=0.5, seed=0)
+    E, _ = body_moduli(info["counts"], 5.0e2, 5.0e3, seed=0)
     # the fluid starts at the inflow velocity: started from rest, the
     # reference's velocity face on this channel becomes unstable after ~140
     # steps (reproduced by the CPU oracle in the reference's operation order)
@@ -294,45 +303,96 @@ def coupled_scene_large(storage="fp32"):
                     initial_velocity=(0.04, 0.0, 0.0),
                     x_left=BoundaryConfig("velocity", velocity=(0.04, 0.0, 0.0)),
                     x_right=BoundaryConfig("pressure", rho=1.0), storage=storage)
-    mpm = MPMConfig(nx=shape[0], ny=shape[1], nz=shape[2], max_particles=len(pos), dx=1.0, dt=1.0 / 112,
-                    density=1.2, youngs_modulus=1.0e3, poisson_ratio=0.3, gravity=(0.0, 0.0, 0.0),
-                    boundary_width=2)
-    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=112)
-    cfg = SimulationConfig(num_steps=100, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
-    return cfg, channel_walls(shape), pos
+    mpm = MPMConfig(nx=shape[0], ny=shape[1], nz=shape[2], max_particles=len(pos), dx=1.0,
+                    dt=1.0 / LARGE_SUBSTEPS, density=1.2, youngs_modulus=1.0e3, poisson_ratio=0.3,
+                    gravity=(0.0, 0.0, 0.0), boundary_width=2)
+    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=LARGE_SUBSTEPS)
+    cfg = SimulationConfig(num_steps=2000, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
+    return cfg, channel_walls(shape), pos, E
 
 
-def time_coupled_large(steps, warmup, storage="fp32", fields=None):
-    """Times the suspension; with `fields` (a dict) also stores the lattice
-    velocity and the particle positions and velocities after the run."""
-    import numpy as np
-    import torch
-
+def large_sim(storage):
     import fsi
 
-    cfg, solid, pos = coupled_scene_large(storage)
+    cfg, solid, pos, E = coupled_scene_large(storage)
     sim = fsi.FSISimulation(cfg)
     sim.initialize_lbm(solid)
     sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
-    for _ in range(warmup):
-        sim.advance()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.advance()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    if fields is not None:
-        # untimed: continue to fields["steps"] coupled steps, then record
-        for _ in range(int(fields.get("steps", 0)) - warmup - steps):
-            sim.advance()
-        fields["u"] = sim.lbm.velocity_numpy().astype(np.float64)
-        fields["x"] = sim.mpm.positions_numpy().astype(np.float64)
-        fields["v"] = sim.mpm.velocities_numpy().astype(np.float64)
+    sim.mpm.set_particle_material(E)
+    return sim, cfg, pos
+
+
+def time_coupled_large(steps, storage="fp32", min_warmup_s=0.5):
+    """Coupled steps/s of the suspension: warm-up of at least two steps and
+    min_warmup_s seconds, then `steps` timed steps."""
+    import torch
+
+    sim, cfg, pos = large_sim(storage)
+    warm_up(sim.advance, 2, min_warmup_s)
+    wall, _ = timed_loop(sim.advance, steps)
     n_sub = cfg.coupling.mpm_substeps_per_lbm_step
-    return {"steps_per_s": round(steps / dt, 3), "ms_per_step": round(1e3 * dt / steps, 3),
+    return {"steps_per_s": round(steps / wall, 3), "ms_per_step": round(1e3 * wall / steps, 3),
             "particles": int(len(pos)), "substeps": n_sub, "timed_steps": steps,
-            "particle_substeps_per_s": round(len(pos) * n_sub * steps / dt, 1)}
+            "particle_substeps_per_s": round(len(pos) * n_sub * steps / wall, 1)}
+
+
+def large_errors_vs_fp64(steps, storages=("fp32", "i16")):
+    """The suspension after `steps` coupled steps from the same start in the
+    FP32 build and the mixed-precision build (I16 lattice populations, FP32
+    MPM and coupling), against the FP64 build: max-norm relative error of the
+    lattice velocity and the particle velocity, the particle position error in
+    cells and relative to the largest displacement, and aggregate measures
+    that a chaotic suspension keeps longer: the mean streamwise velocity, the
+    drag (the particles' step-averaged coupling force summed) and each
+    sphere's centroid."""
+    import numpy as np
+    import torch
+
+    from fsi.scenes import sphere_pack
+
+    _, info = sphere_pack((256, 128, 128), 16, 6.0, 12.0, spacing=0.5, seed=0)
+    bounds = np.concatenate([[0], np.cumsum(info["counts"])])
+
+    def centroids(x):
+        return np.stack([x[bounds[i]:bounds[i + 1]].mean(0) for i in range(len(bounds) - 1)])
+
+    def run(storage):
+        sim, _, pos = large_sim(storage)
+        sim.coupler.record_step_force()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            sim.advance()
+            if (i + 1) % 100 == 0:  # progress of long runs
+                torch.cuda.synchronize()
+                print(f"[suspension {storage}] {i + 1}/{steps} steps, {time.perf_counter() - t0:.0f} s",
+                      file=sys.stderr, flush=True)
+        out = {"u": sim.lbm.velocity_numpy().astype(np.float64), "x": sim.mpm.positions_numpy().astype(np.float64),
+               "v": sim.mpm.velocities_numpy().astype(np.float64), "x0": np.asarray(pos, np.float64),
+               "drag": sim.coupler.step_force_numpy().astype(np.float64).sum(0)}
+        fluid = sim.lbm.solid_numpy() == 0
+        out["mean_ux"] = float(out["u"][..., 0][fluid].mean())
+        out["centroids"] = centroids(out["x"])
+        del sim
+        torch.cuda.empty_cache()
+        return out
+
+    ref = run("fp64")
+    disp = np.abs(ref["x"] - ref["x0"]).max()
+    res = {"steps": steps, "reference": "FP64 build of the same kernels, same scene and steps",
+           "u_max": float(np.abs(ref["u"]).max()), "particle_displacement_max_cells": float(disp)}
+    for storage in storages:
+        got = run(storage)
+        res[storage] = {
+            "u_rel_max": float(np.abs(got["u"] - ref["u"]).max() / max(np.abs(ref["u"]).max(), 1e-30)),
+            "particle_v_rel_max": float(np.abs(got["v"] - ref["v"]).max() / max(np.abs(ref["v"]).max(), 1e-30)),
+            "particle_x_abs_max_cells": float(np.abs(got["x"] - ref["x"]).max()),
+            "particle_x_rel_to_max_displacement": float(np.abs(got["x"] - ref["x"]).max() / max(disp, 1e-30)),
+            "mean_ux_rel": float(abs(got["mean_ux"] - ref["mean_ux"]) / abs(ref["mean_ux"])),
+            "drag_x_rel": float(abs(got["drag"][0] - ref["drag"][0]) / abs(ref["drag"][0])),
+            "sphere_centroid_abs_max_cells": float(np.abs(got["centroids"] - ref["centroids"]).max()),
+        }
+    res["sphere_centroid_displacement_max_cells"] = float(np.abs(ref["centroids"] - centroids(ref["x0"])).max())
+    return res
 
 
 def time_closure(train_scenes=8, train_steps=40, eval_steps=60, large_eval_steps=40, speed_steps=2):
@@ -455,8 +515,12 @@ def main():
                     help="steps of the (untimed) lattice accuracy runs against the FP64 build")
     ap.add_argument("--cpu-task", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-budget", type=float, default=10.0, help=argparse.SUPPRESS)
-    ap.add_argument("--large-steps", type=int, default=3,
-                    help="timed steps of the 256x128x128 coupled suspension (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=10,
+                    help="timed steps of the 256x128x128 coupled suspension, FP32 and mixed (0: skip)")
+    ap.add_argument("--large-error-steps", type=int, default=100,
+                    help="coupled steps of the (untimed) suspension runs compared with its FP64 build")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the suspension: timing and the FP64 comparison (its own JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of one population buffer (counter calibration)")
     args = ap.parse_args()
@@ -470,6 +534,14 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy(args.size)
+        return
+    if args.large_only:
+        out = {"workload": f"256x128x128 coupled suspension, 16 elastic spheres (E log-uniform [5e2, 5e3] per "
+                           f"sphere), {LARGE_SUBSTEPS} substeps",
+               "fp32": time_coupled_large(args.large_steps), "mixed_i16": time_coupled_large(args.large_steps, "i16")}
+        if args.large_error_steps > 0:
+            out["error_vs_fp64"] = large_errors_vs_fp64(args.large_error_steps)
+        print(json.dumps(out), flush=True)
         return
 
     import numpy as np
@@ -572,26 +644,18 @@ def main():
         line["coupled"] = {"workload": "32^3 coupled penalty scene, FP32, 4096 particles, 112 substeps",
                            **time_coupled(args.coupled_steps, 2)}
     if args.large_steps > 0:
-        check = not args.no_error_check
-        f32, f16 = ({"steps": 200}, {"steps": 200}) if check else (None, None)
         line["coupled_large"] = {
-            "workload": "256x128x128 coupled suspension, FP32, 16 elastic spheres, 112 substeps",
-            **time_coupled_large(args.large_steps, 1, fields=f32)}
-        # mixed precision: FP16 shifted populations, FP32 MPM and coupling
-        line["coupled_large_fp16"] = {
-            "workload": "256x128x128 coupled suspension, FP16 lattice storage, FP32 MPM, 112 substeps",
-            **time_coupled_large(args.large_steps, 1, storage="fp16", fields=f16)}
-        # the mixed-precision run against the FP32 run after the same steps
-        # (max-norm relative error per field; particle positions relative to
-        # the lattice extent, displacements relative to the largest one)
-        disp = np.abs(f32["x"] - np.asarray(coupled_scene_large()[2], np.float64)).max() if check else 0.0
-        line["coupled_large_fp16"]["error_vs_fp32"] = None if not check else {
-            "steps": f32["steps"],
-            "u_rel_max": float(np.abs(f16["u"] - f32["u"]).max() / max(np.abs(f32["u"]).max(), 1e-30)),
-            "particle_v_rel_max": float(np.abs(f16["v"] - f32["v"]).max() / max(np.abs(f32["v"]).max(), 1e-30)),
-            "particle_x_abs_max_cells": float(np.abs(f16["x"] - f32["x"]).max()),
-            "particle_x_rel_to_max_displacement": float(np.abs(f16["x"] - f32["x"]).max() / max(disp, 1e-30)),
-        }
+            "workload": f"256x128x128 coupled suspension, 16 elastic spheres (E log-uniform [5e2, 5e3] per sphere), "
+                        f"FP32, {LARGE_SUBSTEPS} substeps",
+            **time_coupled_large(args.large_steps)}
+        # mixed precision: I16 lattice populations, FP32 MPM and coupling
+        line["coupled_large_mixed"] = {
+            "workload": f"the same suspension, I16 lattice storage, FP32 MPM and coupling, {LARGE_SUBSTEPS} substeps",
+            **time_coupled_large(args.large_steps, storage="i16")}
+        line["coupled_large_mixed"]["speedup_vs_fp32"] = round(
+            line["coupled_large_mixed"]["steps_per_s"] / line["coupled_large"]["steps_per_s"], 3)
+        if not args.no_error_check and args.large_error_steps > 0:
+            line["coupled_large_error_vs_fp64"] = large_errors_vs_fp64(args.large_error_steps)
     if not args.no_closure:
         line["learned_closure"] = time_closure()
     if not args.no_cpu_baseline:
