@@ -8,6 +8,7 @@
 // Per substep: clear grid -> P2G (+ coupling) -> grid update + G2P (+ contact).
 
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 
@@ -636,7 +637,10 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
                                          int32_t* __restrict__ boxes_out, int32_t* __restrict__ big_flag,
                                          int big_value, const R* __restrict__ closure, const R* rx, const R* rv,
                                          const R* rC) {
-  using TA = TileAcc<R>;
+  // the plane split (small particle sets, bound by one wavefront's latency,
+  // not by the LDS atomics) keeps FP64 tile sums: the fixed-point scale's
+  // extra workgroup reductions would lengthen its critical path
+  using TA = TileAcc<std::conditional_t<PL == 1, R, double>>;
   using A = typename TA::word;
   using P = typename TA::P;
   static_assert(sizeof(A) == Tile::kWordBytes, "tile word size");
