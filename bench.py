@@ -14,33 +14,6 @@ report coupled steps/s, the suspension also against its FP64 build. Inputs
 are resident in HBM before the timed region.
 
 Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage, and the 32^3 coupled
-penalty scene (4096 particles, 112 MPM substeps per lattice step) reports
-coupled steps/s. Inputs are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 """
 
 from __future__ import annotations
@@ -75,39 +48,9 @@ def algorithmic_bytes(n_fluid, n_cells, pop_bytes):
 
 
 def warm_up(step, count, min_seconds=0.25):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""`count` untimed steps, continued until `min_seconds` have passed (the
+    """`count` untimed steps, continued until `min_seconds` have passed (the
     clocks and caches reach their steady state in a fraction of a second; a
-    short count alone under-reports the fast storages)."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    short count alone under-reports the fast storages)."""
     import torch
 
     t0 = time.perf_counter()
@@ -122,38 +65,8 @@ Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 
 
 def timed_loop(step, steps):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""Wall time of exactly `steps` steps between two synchronisations, and
-    the same interval from HIP events on the launch stream."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    """Wall time of exactly `steps` steps between two synchronisations, and
+    the same interval from HIP events on the launch stream."""
     import torch
 
     stream = torch.cuda.current_stream()
@@ -191,42 +104,12 @@ def time_lattice(solver, steps, warmup, launch_samples):
 
 
 def lattice_errors_vs_fp64(n, mask, fields, steps):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""Per-field error of the timed lattice runs (continued, untimed, to
+    """Per-field error of the timed lattice runs (continued, untimed, to
     `steps` steps) against the FP64 build of the same kernels after the same
     number of steps (the FP64 build is pinned to the CPU oracle at 1e-10):
     max-norm relative error of u, its L2-relative error, and the max-norm
     error of rho (absolute and relative to max |rho - rho0| of the FP64 run),
-    computed on the device."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    computed on the device."""
     import torch
 
     import fsi
@@ -255,37 +138,7 @@ Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 
 
 def read_traffic(name):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     path = ROOT / "profiles" / name
     if not path.exists():
         return None
@@ -296,39 +149,9 @@ Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 
 
 def _cpu_task(task, budget_s):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""One CPU measurement of the oracle (C++ restatement of the reference's
+    """One CPU measurement of the oracle (C++ restatement of the reference's
     algorithm in its operation order, FP32), run in a child process so its
-    OpenMP thread count is its own. Prints one JSON object."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    OpenMP thread count is its own. Prints one JSON object."""
     import numpy as np
 
     from fsi.scenes import porous_spheres
@@ -387,40 +210,10 @@ Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 
 
 def cpu_baseline(budget_s=10.0):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""The CPU oracle on the host cores beside the GPU run, in child processes:
+    """The CPU oracle on the host cores beside the GPU run, in child processes:
     the 256^3 lattice on all cores (the headline comparison) and on one core,
     the 32^3 coupled scene on all cores and on one core, and the 256x128x128
-    suspension's coupled step on all cores (extrapolated from a sample)."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    suspension's coupled step on all cores (extrapolated from a sample)."""
     import subprocess
 
     cores = len(os.sched_getaffinity(0))
@@ -465,40 +258,10 @@ Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 
 
 def coupled_scene_32(storage="fp32"):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""32^3 coupled penalty scene: inflow u_x=0.02 (velocity face x-), pressure
+    """32^3 coupled penalty scene: inflow u_x=0.02 (velocity face x-), pressure
     face x+, no-slip walls as a solid shell in y/z, tau=0.6; elastic cube
     [8,16]x[12,20]^2 of 16^3 particles (spacing 0.5, jitter U(-0.125,0.125),
-    seed 0), E=1e3, nu=0.3, density 1.2, gamma=0.5, 112 substeps."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    seed 0), E=1e3, nu=0.3, density 1.2, gamma=0.5, 112 substeps."""
     from fsi.config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
     from fsi.scenes import channel_walls, jittered_box
 
@@ -520,22 +283,7 @@ LARGE_SUBSTEPS = 256
 
 
 def coupled_scene_large(storage="fp32"):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""256x128x128 coupled suspension: inflow u_x=0.04 (velocity face x-,
+    """256x128x128 coupled suspension: inflow u_x=0.04 (velocity face x-,
     fluid starting at that velocity), pressure face x+, no-slip walls as a
     solid shell in y/z, tau=0.56; 16 non-overlapping elastic spheres r U[6,12]
     (sphere_pack seed 0), 8 particles per cell (spacing 0.5, jitter
@@ -544,22 +292,7 @@ Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
     Substeps: the stiffest sphere's P-wave speed sqrt((lam + 2 mu)/rho) is
     74.9 cells per time unit, and mpm_dt <= 0.3 dx / c_p needs >= 250
     substeps per lattice step (256 here).
-    Returns (config, solid mask, positions, per-particle E)."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    Returns (config, solid mask, positions, per-particle E)."""
     from fsi.config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
     from fsi.scenes import body_moduli, channel_walls, sphere_pack
 
@@ -593,38 +326,8 @@ def large_sim(storage):
 
 
 def time_coupled_large(steps, storage="fp32", min_warmup_s=0.5):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""Coupled steps/s of the suspension: warm-up of at least two steps and
-    min_warmup_s seconds, then `steps` timed steps."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    """Coupled steps/s of the suspension: warm-up of at least two steps and
+    min_warmup_s seconds, then `steps` timed steps."""
     import torch
 
     sim, cfg, pos = large_sim(storage)
@@ -637,44 +340,14 @@ Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 
 
 def large_errors_vs_fp64(steps, storages=("fp32", "i16")):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""The suspension after `steps` coupled steps from the same start in the
+    """The suspension after `steps` coupled steps from the same start in the
     FP32 build and the mixed-precision build (I16 lattice populations, FP32
     MPM and coupling), against the FP64 build: max-norm relative error of the
     lattice velocity and the particle velocity, the particle position error in
     cells and relative to the largest displacement, and aggregate measures
     that a chaotic suspension keeps longer: the mean streamwise velocity, the
     drag (the particles' step-averaged coupling force summed) and each
-    sphere's centroid."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    sphere's centroid."""
     import numpy as np
     import torch
 
@@ -726,43 +399,13 @@ Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
 
 
 def time_closure(train_scenes=8, train_steps=40, eval_steps=60, large_eval_steps=40, speed_steps=2):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""Learned closure: trains on coarse/2x-refined pairs (single-ellipsoid
+    """Learned closure: trains on coarse/2x-refined pairs (single-ellipsoid
     channels, 48x24x24 coarse), then on a held-out ellipsoid
     scene and on the 64-sphere channel (192x96x96 coarse, FP16 populations;
     384x192x192 refined, FP32) compares drag and centroid displacement of
     the plain coarse run and of the coarse run with the closure against the
     refined run, and times the coarse run with the closure against the
-    refined run."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    refined run."""
     import numpy as np
     import torch
 
@@ -841,39 +484,9 @@ def time_coupled(steps, warmup):
 
 
 def calibrate_copy(n):
-    """Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""A plain device-to-device copy of one FP32 population buffer
+    """A plain device-to-device copy of one FP32 population buffer
     (19 * n^3 * 4 bytes), three times: the known-size kernel that scales the
-    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""Benchmark of the coupled LBM-MPM step on one MI355X.
-
-Workload (the lattice bandwidth case): D3Q19 MRT on a 256^3 lattice with a
-random porous solid mask (porosity 0.8, spheres of radius 4-10 cells, seed 0),
-all faces periodic, body force g_x = 1e-6, tau = 0.55. A step is one lattice
-update of the whole 256^3 lattice (one fused collide+stream launch); `value`
-is the FP32-population MLUPS (all cells), the reference's arithmetic type.
-The same scene then runs with FP16 shifted storage and with I16 fixed-point
-storage (untimed 1,000-step accuracy runs against the FP64 build), the 32^3
-coupled penalty scene (4096 particles, 112 MPM substeps per lattice step)
-and the 256x128x128 suspension (16 spheres of per-sphere E, 256 substeps)
-report coupled steps/s, the suspension also against its FP64 build. Inputs
-are resident in HBM before the timed region.
-
-Prints ONE JSON line. `--gpus` above 1 is refused: this build runs on one GPU.
-"""
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
     import torch
 
     nbytes = 19 * n ** 3 * 4
