@@ -161,3 +161,17 @@ def test_lbm_load_solid_from_txt(tmp_path):
     np.savetxt(path, solid.reshape(-1, order="F"), fmt="%d")
     got = LBMSolver3D.load_solid_from_txt(str(path), 4, 3, 2)
     assert np.array_equal(got, solid)
+
+
+def test_bench_command_line_on_the_cpu():
+    """bench.py parses its arguments without a GPU and refuses more than one
+    GPU with a message (this build runs on one)."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    bench = Path(__file__).resolve().parents[1] / "bench.py"
+    r = subprocess.run([sys.executable, str(bench), "--help"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "--large-error-steps" in r.stdout
+    r = subprocess.run([sys.executable, str(bench), "--gpus", "8"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
