@@ -279,33 +279,54 @@ class LBMMpmCoupler:
 
     # -------------------------------------------------------- accessors --
 
-    def _cells(self, t: torch.Tensor, ncomp: int) -> np.ndarray:
-        return t.cpu().numpy().reshape(self.nx, self.ny, self.nz, ncomp)
+    # Device accessors: the *_numpy arrays as fresh device tensors (same
+    # shapes, dtypes and particle order); the *_numpy accessors copy them.
 
-    def coupling_force_numpy(self) -> np.ndarray:
-        return np.ascontiguousarray(self._cells(self.cforce, 4)[..., :3])
+    def _cells_device(self, t: torch.Tensor, ncomp: int) -> torch.Tensor:
+        return t.view(self.nx, self.ny, self.nz, ncomp)
 
-    def solid_volume_fraction_numpy(self) -> np.ndarray:
-        return np.ascontiguousarray(self._cells(self.cforce, 4)[..., 3])
+    def coupling_force_device(self) -> torch.Tensor:
+        return self._cells_device(self.cforce, 4)[..., :3].contiguous()
 
-    def immersed_boundary_force_numpy(self) -> np.ndarray:
+    def solid_volume_fraction_device(self) -> torch.Tensor:
+        return self._cells_device(self.cforce, 4)[..., 3].contiguous()
+
+    def immersed_boundary_force_device(self) -> torch.Tensor:
         out = torch.empty_like(self.ib_force)
         nat.call("fsi_soa_to_aos", self.lbm.real_code, self.nx * self.ny * self.nz, 3,
                  nat.ptr(self.ib_force), nat.ptr(out))
-        return self._cells(out, 3)
+        return self._cells_device(out, 3)
 
-    def _per_particle(self, t: torch.Tensor) -> np.ndarray:
+    def _per_particle_device(self, t: torch.Tensor) -> torch.Tensor:
         # slot order -> input order (the particles may have been sorted)
-        return self.mpm._input_order(t[: self.mpm.particle_count()].cpu().numpy().copy())
+        return self.mpm.input_order_device(t[: self.mpm.particle_count()])
+
+    def particle_valid_weights_device(self) -> torch.Tensor:
+        return self._per_particle_device(self.valid_weight)
+
+    def particle_coupling_mask_device(self) -> torch.Tensor:
+        return self._per_particle_device(self.coupling_mask)
+
+    def particle_contact_mask_device(self) -> torch.Tensor:
+        return self._per_particle_device(self.contact_mask)
+
+    def coupling_force_numpy(self) -> np.ndarray:
+        return self.coupling_force_device().cpu().numpy()
+
+    def solid_volume_fraction_numpy(self) -> np.ndarray:
+        return self.solid_volume_fraction_device().cpu().numpy()
+
+    def immersed_boundary_force_numpy(self) -> np.ndarray:
+        return self.immersed_boundary_force_device().cpu().numpy()
 
     def particle_valid_weights_numpy(self) -> np.ndarray:
-        return self._per_particle(self.valid_weight)
+        return self.particle_valid_weights_device().cpu().numpy()
 
     def particle_coupling_mask_numpy(self) -> np.ndarray:
-        return self._per_particle(self.coupling_mask)
+        return self.particle_coupling_mask_device().cpu().numpy()
 
     def particle_contact_mask_numpy(self) -> np.ndarray:
-        return self._per_particle(self.contact_mask)
+        return self.particle_contact_mask_device().cpu().numpy()
 
     # The accessors keep the reference's float32 host reductions;
     # diagnostic_vector() gathers the FP64 device reductions the per-step

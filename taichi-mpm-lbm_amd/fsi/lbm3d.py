@@ -387,21 +387,37 @@ class LBMSolver3D:
 
     # -------------------------------------------------------- accessors --
 
-    def _aos(self, soa: torch.Tensor, ncomp: int) -> np.ndarray:
+    def _aos_device(self, soa: torch.Tensor, ncomp: int) -> torch.Tensor:
         out = torch.empty(soa.numel(), dtype=soa.dtype, device=self.device)
         nat.call("fsi_soa_to_aos", self.real_code, self.n_cells, ncomp, nat.ptr(soa), nat.ptr(out))
-        return out.cpu().numpy().reshape(self.nx, self.ny, self.nz, ncomp)
+        return out.view(self.nx, self.ny, self.nz, ncomp)
+
+    # Device accessors: the *_numpy arrays' contents, shapes and dtypes as
+    # device tensors (fresh tensors or views of the state: copy before the
+    # next step changes them); the *_numpy accessors copy them to the host.
+
+    def density_device(self) -> torch.Tensor:
+        self._ensure_macro()
+        return self.rho.view(self.nx, self.ny, self.nz)
+
+    def velocity_device(self) -> torch.Tensor:
+        self._ensure_macro()
+        return self._aos_device(self.vel, 3)
+
+    def force_device(self) -> torch.Tensor:
+        return self._aos_device(self.force, 3)
+
+    def solid_device(self) -> torch.Tensor:
+        return self.solid.view(self.nx, self.ny, self.nz)
 
     def density_numpy(self) -> np.ndarray:
-        self._ensure_macro()
-        return self.rho.cpu().numpy().reshape(self.nx, self.ny, self.nz)
+        return self.density_device().cpu().numpy()
 
     def velocity_numpy(self) -> np.ndarray:
-        self._ensure_macro()
-        return self._aos(self.vel, 3)
+        return self.velocity_device().cpu().numpy()
 
     def solid_numpy(self) -> np.ndarray:
-        return self.solid.cpu().numpy().reshape(self.nx, self.ny, self.nz)
+        return self.solid_device().cpu().numpy()
 
     def distribution_numpy(self) -> np.ndarray:
         self._ensure_macro()
@@ -413,7 +429,7 @@ class LBMSolver3D:
         return out.cpu().numpy().reshape(self.nx, self.ny, self.nz, _Q)
 
     def force_numpy(self) -> np.ndarray:
-        return self._aos(self.force, 3)
+        return self.force_device().cpu().numpy()
 
     def lattice_sums(self, cforce: torch.Tensor | None = None) -> torch.Tensor:
         """FP64 sums over the materialised state, on the device (5 doubles):

@@ -387,27 +387,54 @@ class MPMSolver3D:
 
     # --------------------------------------------------------- accessors --
 
-    def _plane(self, t: torch.Tensor, ncomp: int) -> np.ndarray:
+    def input_order_device(self, rows: torch.Tensor) -> torch.Tensor:
+        """Rows of a slot-ordered device tensor (first particle_count()
+        slots) in input order, as a fresh contiguous tensor."""
+        if not self._permuted:
+            return rows.contiguous()
+        out = torch.empty(rows.shape, dtype=rows.dtype, device=rows.device)
+        out.index_copy_(0, self.ids[: self.num_particles].long(), rows)
+        return out
+
+    def _plane_device(self, t: torch.Tensor, ncomp: int) -> torch.Tensor:
         n = self.num_particles
-        return self._input_order(t.view(ncomp, self.max_particles)[:, :n].cpu().numpy().T.copy())
+        return self.input_order_device(t.view(ncomp, self.max_particles)[:, :n].t())
 
     def particle_count(self) -> int:
         return int(self.num_particles)
 
+    # Device accessors: the *_numpy arrays as fresh device tensors (input
+    # order, same shapes and dtypes); the *_numpy accessors copy them.
+
+    def positions_device(self) -> torch.Tensor:
+        return self._plane_device(self.x, 3)
+
+    def velocities_device(self) -> torch.Tensor:
+        return self._plane_device(self.v, 3)
+
+    def particle_forces_device(self) -> torch.Tensor:
+        return self._plane_device(self.particle_force, 3)
+
+    def deformation_gradients_device(self) -> torch.Tensor:
+        return self._plane_device(self.F, 9).view(-1, 3, 3)
+
+    def active_device(self) -> torch.Tensor:
+        return self.input_order_device(self.active[: self.num_particles])
+
     def positions_numpy(self) -> np.ndarray:
-        return self._plane(self.x, 3)
+        return self.positions_device().cpu().numpy()
 
     def velocities_numpy(self) -> np.ndarray:
-        return self._plane(self.v, 3)
+        return self.velocities_device().cpu().numpy()
 
     def particle_forces_numpy(self) -> np.ndarray:
-        return self._plane(self.particle_force, 3)
+        return self.particle_forces_device().cpu().numpy()
 
     def deformation_gradients_numpy(self) -> np.ndarray:
-        return self._plane(self.F, 9).reshape(-1, 3, 3)
+        return self.deformation_gradients_device().cpu().numpy()
 
     def active_numpy(self) -> np.ndarray:
-        return self._input_order(self.active[: self.num_particles].cpu().numpy().copy())
+        return self.active_device().cpu().numpy()
 
     def masses_numpy(self) -> np.ndarray:
         return self._input_order(self.mass[: self.num_particles].cpu().numpy().copy())
