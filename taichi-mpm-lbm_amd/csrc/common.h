@@ -124,6 +124,26 @@ template <> struct PopStore<int16_t> {
   __device__ static int16_t store(float x) { return __builtin_amdgcn_cvt_pknorm_i16(x, 0.0f)[0]; }
 };
 
+// Largest of a non-negative float over the 64 lanes (all active), the same
+// value in every lane: DPP row shifts make lane 15 of each row the row's
+// maximum, two row broadcasts carry rows 0-2 into lane 63, which is read.
+// Non-negative floats order as their bit patterns.
+template <int CTRL, int ROW_MASK> __device__ __forceinline__ int dpp_max_step(int b) {
+  const int o = __builtin_amdgcn_update_dpp(b, b, CTRL, ROW_MASK, 0xf, false);
+  return (int)max((uint32_t)b, (uint32_t)o);
+}
+
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  int b = (int)__float_as_uint(v);
+  b = dpp_max_step<0x111, 0xf>(b);  // row_shr:1
+  b = dpp_max_step<0x112, 0xf>(b);  // row_shr:2
+  b = dpp_max_step<0x114, 0xf>(b);  // row_shr:4
+  b = dpp_max_step<0x118, 0xf>(b);  // row_shr:8
+  b = dpp_max_step<0x142, 0xa>(b);  // row_bcast:15 into rows 1 and 3
+  b = dpp_max_step<0x143, 0xc>(b);  // row_bcast:31 into rows 2 and 3
+  return __uint_as_float((uint32_t)__builtin_amdgcn_readlane(b, 63));
+}
+
 // weight class of a D3Q19 direction: 0 rest, 1 axis, 2 diagonal
 __host__ __device__ constexpr int wclass(int q) { return q == 0 ? 0 : (q < 7 ? 1 : 2); }
 

@@ -467,11 +467,15 @@ template <typename R, bool COUPLE> constexpr int p2g_lds_bytes() {
 // addresses costs ~28 clocks as ds_add_f64, ~16 as ds_add_u64 and ~196 as
 // ds_add_f32 (processed lane by lane), and the tile atomics are most of the
 // fused substep. Each workgroup scales its FP32 contributions by a power of
-// two 2^S chosen from the largest magnitude any of its lanes adds (TileScale),
-// so every node sum of its <= 256 particles fits a signed 64-bit integer; a
-// contribution keeps its FP32 product's precision down to 2^-S ~ 2^-54 of
-// that magnitude, and the sums are exact (independent of the order of the
-// atomics). The flush converts each sum back once per node and value.
+// two 2^S chosen from the largest magnitude any of its lanes adds (TileScale)
+// so that every scaled contribution is below 2^30 in magnitude; it is rounded
+// to an integer (v_cvt_rpi_i32_f32: floor(c + 1/2)) and added as a signed
+// 64-bit word, so a node sum of any number of contributions below 2^33 is
+// exact and independent of the order of the atomics. A contribution keeps
+// 2^-30 of the workgroup's largest contribution, below the rounding of the
+// FP32 product itself (2^-24 of its own magnitude) wherever the product is
+// within 2^6 of that largest value. The flush converts each sum back once
+// per node and value.
 template <typename R> struct TileAcc;
 template <> struct TileAcc<double> {
   using word = double;
@@ -484,13 +488,12 @@ template <> struct TileAcc<float> {
   using word = unsigned long long;
   using P = float;
   static constexpr bool fixed = true;
-  // c: the contribution times 2^(S-32), |c| < 2^22. Two's complement
-  // floor(c) * 2^32 + fract(c) * 2^32 (the fraction truncated to 32 bits).
+  // c: the contribution times 2^S, |c| < 2^30: one rounding conversion and
+  // the sign extension
   __device__ static void add(word* a, float c) {
-    int32_t hi;  // floor(c), one instruction (the compiler emits floor + convert)
-    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(hi) : "v"(c));
-    const uint32_t lo = (uint32_t)(__builtin_amdgcn_fractf(c) * 4294967296.0f);
-    atomicAdd(a, ((word)(uint32_t)hi << 32) | (word)lo);
+    int32_t r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(c));
+    atomicAdd(a, (word)(long long)r);
   }
   // sum * 2^-S (inv = 2^-S; NaN when the workgroup added a non-finite value)
   __device__ static float value(word s, double inv) {
@@ -520,39 +523,52 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// Power-of-two scale of one batch of a workgroup's tile contributions (all
-// contributions of the batch have |c| <= the workgroup maximum of `bound`):
-// mul = 2^(S-32) applied to each FP32 contribution before TileAcc::add, inv =
-// 2^-S for the flush. Every thread calls it (it synchronises the workgroup).
+// Power-of-two scales of a workgroup's tile contributions, one per tile
+// value (momentum xyz and mass, or coupling force xyz and phi: quantities of
+// different magnitudes): every contribution to value a has |c| <= the
+// workgroup maximum of bound[a]. mul[a] = 2^S_a is applied to each FP32
+// contribution before TileAcc::add, inv[a] = 2^-S_a at the flush. Every
+// thread calls it (it synchronises the workgroup).
 struct TileScale {
-  float mul;
-  double inv;
+  float mul[4];
+  double inv[4];
 };
 template <int NW>
-__device__ __forceinline__ TileScale tile_scale(float bound, uint32_t* red) {
+__device__ __forceinline__ TileScale tile_scale(const float bound[4], uint32_t (*red)[NW]) {
   // nonnegative floats order as their bit patterns; NaN (0x7fc00000) and
   // inf order above every finite value
-  const float b = bound >= 0.0f ? bound : (bound == bound ? -bound : __int_as_float(0x7fc00000));
-  const uint32_t m = (uint32_t)wave_max((int)__float_as_uint(b));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  uint32_t w = red[0];
+  uint32_t m[4];
 #pragma unroll
-  for (int i = 1; i < NW; ++i) w = max(w, red[i]);
-  const float mx = __uint_as_float(w);
-  TileScale s;
-  if (!(mx < __builtin_huge_valf())) {  // inf or NaN: sums become NaN at the flush
-    s.mul = 0.0f;
-    s.inv = __builtin_nan("");
-    return s;
+  for (int a = 0; a < 4; ++a) {
+    const float b = bound[a] >= 0.0f ? bound[a] : (bound[a] == bound[a] ? -bound[a] : __int_as_float(0x7fc00000));
+    m[a] = __float_as_uint(wave_max_nonneg(b));
   }
-  // mx < 2^e; 256 contributions per node sum below 2^(e+8); S = 54 - e keeps
-  // the sums below 2^62 and each scaled contribution below 2^22
-  int e = 0;
-  (void)frexpf(mx, &e);
-  e = min(max(e, -100), 100);
-  s.mul = ldexpf(1.0f, 22 - e);
-  s.inv = ldexp(1.0, e - 54);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) red[a][threadIdx.x >> 6] = m[a];
+  }
+  __syncthreads();
+  TileScale s;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    uint32_t w = red[a][0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) w = max(w, red[a][i]);
+    const float mx = __uint_as_float(w);
+    if (!(mx < __builtin_huge_valf())) {  // inf or NaN: sums become NaN at the flush
+      s.mul[a] = 0.0f;
+      s.inv[a] = __builtin_nan("");
+      continue;
+    }
+    // mx < 2^e; S = 30 - e keeps each scaled contribution below 2^30 (and
+    // its rounding in int32); 27 contributions per particle of a workgroup
+    // of <= 256 particles sum far below 2^63
+    int e = 0;
+    (void)frexpf(mx, &e);
+    e = min(max(e, -96), 100);  // 2^(30 - e) stays a finite float
+    s.mul[a] = ldexpf(1.0f, 30 - e);
+    s.inv[a] = ldexp(1.0, e - 30);
+  }
   return s;
 }
 
@@ -653,8 +669,8 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   constexpr int kS = Slots<PL>::value;
   __shared__ int red[6][PL * kS / 64];
   // fixed-point tile: maxima of the coupling and the P2G contributions
-  __shared__ uint32_t sred[2][PL * kS / 64];
-  TileScale cscale{1.0f, 1.0}, pscale{1.0f, 1.0};
+  __shared__ uint32_t sred[2][4][PL * kS / 64];
+  TileScale cscale{{1.0f, 1.0f, 1.0f, 1.0f}, {1.0, 1.0, 1.0, 1.0}}, pscale = cscale;
   // plane split (PL = 3): partial support and interpolated velocity
   __shared__ R cpart[PL > 1 && COUPLE ? PL : 1][PL > 1 && COUPLE ? kS : 1][4];
   static_assert(PL == 1 || (PL == 3 && sizeof(R) == 4), "plane split: FP32 only");
@@ -970,12 +986,12 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     }
     if (use_lds) {  // uniform over the workgroup
       if constexpr (TA::fixed) {
-        const float bnd = spreads ? fmaxf(fmaxf(fabsf((float)rfn[0]), fabsf((float)rfn[1])),
-                                          fmaxf(fabsf((float)rfn[2]), cp.build ? fabsf((float)rfn[3]) : 0.0f))
-                                  : 0.0f;
+        float bnd[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) bnd[k] = spreads && (k < 3 || cp.build) ? fabsf((float)rfn[k]) : 0.0f;
         cscale = tile_scale<PL * kS / 64>(bnd, sred[0]);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) rfn[k] *= cscale.mul;
+        for (int k = 0; k < 4; ++k) rfn[k] *= cscale.mul[k];
       }
       if (spreads) {
         // all 27 nodes in the lane-rotated order; the values of invalid
@@ -1079,21 +1095,19 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
       }
     }
     if (use_lds) {  // uniform over the workgroup
-      float mul = 1.0f;
       if constexpr (TA::fixed) {
-        // largest |mom0 + aff dp| over the stencil: |dp| <= 1.5 dx per axis
-        float bnd = 0.0f;
+        // largest |mom0 + aff dp| per component over the stencil (|dp| <=
+        // 1.5 dx per axis), and the mass
+        float bnd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (scatters) {
-          bnd = fabsf((float)mp);
 #pragma unroll
           for (int a = 0; a < 3; ++a)
-            bnd = fmaxf(bnd, fabsf((float)mom0[a]) +
-                                 1.5f * (float)m.dx *
-                                     (fabsf((float)aff[a * 3]) + fabsf((float)aff[a * 3 + 1]) +
-                                      fabsf((float)aff[a * 3 + 2])));
+            bnd[a] = fabsf((float)mom0[a]) + 1.5f * (float)m.dx *
+                                                 (fabsf((float)aff[a * 3]) + fabsf((float)aff[a * 3 + 1]) +
+                                                  fabsf((float)aff[a * 3 + 2]));
+          bnd[3] = fabsf((float)mp);
         }
         pscale = tile_scale<PL * kS / 64>(bnd, sred[1]);
-        mul = pscale.mul;
       }
       if (scatters) {
         // lane-rotated order as for the coupling scatter; the affine
@@ -1119,6 +1133,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
           cy[k] = P(pick4(c, aff[1], aff[4], aff[7], R(0)));
           cz[k] = P(pick4(c, aff[2], aff[5], aff[8], R(0)));
           if constexpr (TA::fixed) {
+            const P mul = P(pick4(c, pscale.mul[0], pscale.mul[1], pscale.mul[2], pscale.mul[3]));
             cm[k] *= mul;
             cx[k] *= mul;
             cy[k] *= mul;
@@ -1164,14 +1179,14 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const A sum = tile[s * kN + t];
-        if (sum != A(0)) atomicAdd(grid + 4 * node + s, R(TA::value(sum, pscale.inv)));
+        if (sum != A(0)) atomicAdd(grid + 4 * node + s, R(TA::value(sum, pscale.inv[s])));
       }
       if constexpr (COUPLE) {
         if (ctile[t][3] != R(0)) {
 #pragma unroll
           for (int s = 4; s < Tile::kValues; ++s) {
             const A sum = tile[s * kN + t];
-            if (sum != A(0)) atomicAdd(cforce + 4 * node + (s - 4), R(TA::value(sum, cscale.inv)));
+            if (sum != A(0)) atomicAdd(cforce + 4 * node + (s - 4), R(TA::value(sum, cscale.inv[s - 4])));
           }
         }
       }
@@ -1412,28 +1427,49 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
         for (int c = 0; c < 3; ++c) nc[r * 3 + c] += s * (gv[r] * dp[c]);
     };
     if (staged) {
+      // the 27-node sums in separable form: per (i, j) row of the stencil
+      // the k-sums A = sum_k w_k v and B = sum_k w_k (k - fx_z) v, then
+      // v += w_i w_j A and T = sum w v dp^T by columns: w_i w_j (i - fx_x) A,
+      // w_i w_j (j - fx_y) A, w_i w_j B; C = 4 inv_dx T. A third of the
+      // per-node products of the direct form.
+      R wkz[3], T[9];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) wkz[k] = w[k][2] * (R(k) - fx[2]);
+#pragma unroll
+      for (int r = 0; r < 9; ++r) T[r] = R(0);
+      auto row = [&](int i, R wi, int j) {
+        const R(*tv)[3] = vtile + box.index(base[0] + i, base[1] + j, base[2]);
+        R A[3] = {R(0), R(0), R(0)}, B[3] = {R(0), R(0), R(0)};
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const R g = tv[k][a];
+            A[a] += w[k][2] * g;
+            B[a] += wkz[k] * g;
+          }
+        const R wij = wi * w[j][1];
+        const R ax = wij * (R(i) - fx[0]), ay = wij * (R(j) - fx[1]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          nv[a] += wij * A[a];
+          T[a * 3 + 0] += ax * A[a];
+          T[a * 3 + 1] += ay * A[a];
+          T[a * 3 + 2] += wij * B[a];
+        }
+      };
       if constexpr (PL > 1) {
         const R wi = pick3(plane, w[0][0], w[1][0], w[2][0]);
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            const R* tv = vtile[box.index(base[0] + plane, base[1] + j, base[2] + k)];
-            const R gv[3] = {tv[0], tv[1], tv[2]};
-            accumulate(plane, wi, j, k, gv);
-          }
+        for (int j = 0; j < 3; ++j) row(plane, wi, j);
       } else {
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              const R* tv = vtile[box.index(base[0] + i, base[1] + j, base[2] + k)];
-              const R gv[3] = {tv[0], tv[1], tv[2]};
-              accumulate(i, w[i][0], j, k, gv);
-            }
+          for (int j = 0; j < 3; ++j) row(i, w[i][0], j);
       }
+#pragma unroll
+      for (int r = 0; r < 9; ++r) nc[r] = m.four_inv_dx * T[r];
     } else {
       // box too large for LDS (particles far apart): gather from the grid in
       // batches of NB i-planes, all loads of a batch in flight together (one
