@@ -54,3 +54,32 @@ def test_error_measures():
     assert e["drag_rel_err"] == pytest.approx(0.1) and e["drag_rel_err_second_half"] == pytest.approx(0.1)
     # vector error: |(0, 0.1, 0)| / |(1, 0, 0)| for body 0, 0 for body 1
     assert e["displacement_rel_err"] == pytest.approx(0.05) and e["displacement_rel_err_max"] == pytest.approx(0.1)
+
+
+def test_stable_substeps_hold_the_p_wave_courant_number():
+    """S = ceil(c_p / 0.3) with c_p = sqrt(E (1 - nu) / ((1 + nu)(1 - 2 nu)) / rho),
+    at least 112: the 1e3 material of the coupled scenes needs 112 and the
+    stiffest sphere of the suspension (5e3) 250, as their configurations
+    state; the refined run of a pair keeps the count per lattice step."""
+    from fsi.closure import stable_substeps
+
+    assert stable_substeps(1.0e3, 0.3, 1.2) == 112
+    assert stable_substeps(5.0e3, 0.3, 1.2) == 250
+    assert stable_substeps(2.0e2, 0.2, 1.2) == 112  # soft material: the floor
+    for E, nu in ((2.0e4, 0.4), (7.3e3, 0.25), (3.1e3, 0.33)):
+        s = stable_substeps(E, nu, 1.2)
+        c = np.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) / 1.2)
+        assert c / s <= 0.3 < c / (s - 1)
+    sc = ClosureScene.random(1013, shape=(96, 48, 48), axes=(4.0, 10.0), modulus=(2.0e2, 2.0e4), substeps="stable")
+    assert sc.substeps == stable_substeps(sc.youngs_modulus, sc.poisson_ratio, sc.density)
+    assert sc.config(refined=True).coupling.mpm_substeps_per_lbm_step == sc.substeps
+
+
+def test_pack_scene_per_body_moduli():
+    """Per-sphere E log-uniform in the range, one value per body, and the
+    configuration's substeps taken from the stiffest."""
+    sc = PackScene(shape=(48, 24, 24), count=4, radii=(2.5, 4.0), seed=3, body_modulus=(5.0e2, 5.0e3))
+    E = sc.body_moduli()
+    assert len(E) == len(sc.bodies()) and np.all((E >= 5.0e2) & (E <= 5.0e3)) and len(set(E.tolist())) == len(E)
+    assert sc.config().mpm.youngs_modulus == pytest.approx(E.max())
+    assert PackScene(shape=(48, 24, 24), count=4, radii=(2.5, 4.0), seed=3).body_moduli() is None
