@@ -462,6 +462,101 @@ def time_closure(train_scenes=8, train_steps=40, eval_steps=60, large_eval_steps
                       "closure_fallbacks": coarse.coupler.closure_fallback_count()}}
 
 
+CFG4_SCENES = dict(shape=(96, 48, 48), axes=(4.0, 10.0), modulus=(2.0e2, 2.0e4), poisson=(0.2, 0.4),
+                   inflow=(0.01, 0.05), substeps="stable")
+
+
+def closure_data_and_training(n_scenes, steps=1500, every=50, warmup=50, adam_steps=3000, eval_scenes=2,
+                              save_path=None):
+    """The closure's data set and training at the sizes of its configuration:
+    coarse 96x48x48 against 2x-refined 192x96x96 runs of single-ellipsoid
+    channels (semi-axes U[4, 10] coarse cells, random orientation, E
+    log-uniform in [2e2, 2e4], nu U[0.2, 0.4], inflow U[0.01, 0.05]; each
+    scene at the substep count its material needs), `steps` coarse steps per
+    scene, features and targets sampled every `every` coarse steps after
+    `warmup`; MLP trained with batch 65,536 and Adam lr 1e-3; then plain and
+    closure coarse runs of held-out scenes against their refined runs over
+    the same horizon. Prints progress on stderr."""
+    import numpy as np
+    import torch
+
+    from fsi.closure import ClosureScene, accuracy_vs_refined, scene_pair_data, train_closure
+
+    scenes = [ClosureScene.random(s, **CFG4_SCENES) for s in range(1000, 1000 + n_scenes)]
+    t0 = time.perf_counter()
+    Xs, Ts = [], []
+    for i, sc in enumerate(scenes):
+        X, T = scene_pair_data(sc, steps, every=every, warmup=warmup)
+        Xs.append(X)
+        Ts.append(T)
+        print(f"[closure data] scene {i + 1}/{n_scenes}: {len(X)} rows, {sc.substeps} substeps, "
+              f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    X = np.concatenate(Xs)
+    T = np.concatenate(Ts)
+    t1 = time.perf_counter()
+    closure, hist = train_closure(X, T, steps=adam_steps, batch=65536, lr=1.0e-3, seed=1)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    if save_path:
+        closure.save(save_path)
+    held = {}
+    for s in range(2000, 2000 + eval_scenes):
+        sc = ClosureScene.random(s, **CFG4_SCENES)
+        held[f"seed {s}"] = {"semi_axes": [round(a, 2) for a in sc.semi_axes], "youngs_modulus": round(sc.youngs_modulus, 1),
+                             "inflow": round(sc.inflow, 4), "substeps": sc.substeps,
+                             **accuracy_vs_refined(sc, steps, closure, force_every=every, progress=f"held-out {s}")}
+    return {"workload": f"closure data: {n_scenes} seeded single-ellipsoid scenes, 96x48x48 coarse vs 192x96x96 "
+                        f"refined, {steps} coarse steps each (samples every {every} after {warmup})",
+            "substeps": [sc.substeps for sc in scenes],
+            "train": {"samples": int(len(X)), "batch": 65536, "lr": 1.0e-3, "adam_steps": adam_steps,
+                      "data_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 2),
+                      "loss_first": round(hist[0], 4), "loss_last": round(float(np.mean(hist[-50:])), 4)},
+            "held_out": held, "closure_file": str(save_path) if save_path else None}
+
+
+def closure_large_scene(closure_path, steps=2000, speed_steps=2):
+    """The large coupled scene of the closure's configuration: 64 elastic
+    spheres (radius 5-14 refined cells, E log-uniform in [5e2, 5e3] per
+    sphere, nu 0.3) in a 384x192x192 channel (FP32, plain penalty coupling),
+    `steps` coarse steps = 2 x `steps` refined steps, against the 192x96x96
+    run with FP16 populations, plain and with the closure saved at
+    `closure_path`: drag, centroid displacement, per-particle force errors,
+    fallback counts, and the time per coarse step of the closure run against
+    the refined run. Prints progress on stderr."""
+    import torch
+
+    from fsi.closure import LearnedClosure, PackScene, accuracy_vs_refined, stable_substeps
+
+    scene = PackScene(shape=(192, 96, 96), count=64, radii=(2.5, 7.0), poisson_ratio=0.3, inflow=0.04,
+                      margin=4.0, gap=2.0, seed=5, body_modulus=(5.0e2, 5.0e3),
+                      substeps=stable_substeps(5.0e3, 0.3, 1.2))
+    closure = LearnedClosure.load(closure_path) if closure_path else None
+    acc = accuracy_vs_refined(scene, steps, closure, storage="fp16", force_every=50, progress="large scene")
+    coarse = scene.simulation(False, "fp16")
+    if closure is not None:
+        coarse.coupler.set_closure(closure)
+    fine = scene.simulation(True)
+
+    def timed(sim, n):
+        sim.step()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(n):
+            sim.step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / n
+
+    tc = timed(coarse, speed_steps)
+    tf = timed(fine, 2 * speed_steps) * 2.0
+    return {"workload": f"64 spheres (r 5-14 refined cells, E log-uniform [5e2, 5e3] per sphere), 384x192x192 "
+                        f"FP32 refined ({2 * steps} steps) vs 192x96x96 FP16 coarse ({steps} steps), "
+                        f"{scene.substeps} substeps per lattice step",
+            "coarse_particles": int(coarse.mpm.num_particles), "refined_particles": int(fine.mpm.num_particles),
+            "accuracy_vs_refined": acc,
+            "speed": {"coarse_ms_per_step": round(1e3 * tc, 3), "refined_ms_per_coarse_step": round(1e3 * tf, 3),
+                      "speedup": round(tf / tc, 2)}}
+
+
 def time_coupled(steps, warmup):
     import torch
 
@@ -524,6 +619,13 @@ def main():
                     help="coupled steps of the (untimed) suspension runs compared with its FP64 build")
     ap.add_argument("--large-only", action="store_true",
                     help="only the suspension: timing and the FP64 comparison (its own JSON line)")
+    ap.add_argument("--closure-config", choices=("train", "large"), default=None,
+                    help="only the learned closure at its configuration's sizes (its own JSON line): 'train' "
+                         "generates the coarse/refined pair data and trains; 'large' runs the 64-sphere scene")
+    ap.add_argument("--closure-scenes", type=int, default=32, help="scenes of --closure-config train")
+    ap.add_argument("--closure-steps", type=int, default=0,
+                    help="coarse steps per run of --closure-config (default 1500 train, 2000 large)")
+    ap.add_argument("--closure-path", default=None, help="closure file written by train and read by large")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of one population buffer (counter calibration)")
     args = ap.parse_args()
@@ -537,6 +639,14 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy(args.size)
+        return
+    if args.closure_config == "train":
+        out = closure_data_and_training(args.closure_scenes, steps=args.closure_steps or 1500,
+                                        save_path=args.closure_path)
+        print(json.dumps(out), flush=True)
+        return
+    if args.closure_config == "large":
+        print(json.dumps(closure_large_scene(args.closure_path, steps=args.closure_steps or 2000)), flush=True)
         return
     if args.large_only:
         out = {"workload": f"256x128x128 coupled suspension, 16 elastic spheres (E log-uniform [5e2, 5e3] per "

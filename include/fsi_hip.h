@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FSI_ABI_VERSION 3
+#define FSI_ABI_VERSION 4
 
 enum {
   FSI_OK = 0,
@@ -368,6 +368,20 @@ int fsi_force_assemble(const fsi_lbm_params* lp, const fsi_lbm_buffers* lb,
                        const fsi_force_assembly* a, void* cforce, void* ib_force, void* ib_total,
                        int32_t* counters, int32_t write_force, void* stream);
 
+/* LBMMpmCoupler.step tail fused with the next lattice collision,
+ * fsi/coupling.py:162-170 (clamp phi, reset the force, add the coupling
+ * force, compute and add the IB drag) + fsi/lbm3d.py:314-333 (collide): from
+ * the materialised state (pop_in, b->rho, b->vel) each cell assembles its
+ * force from cforce (and, with IB, rho and u), writes it to b->force (and the
+ * drag to ib_force, summed into ib_total, counted in counters[5..6]) and
+ * collides with it into pop_out, the post-collision state of
+ * fsi_lbm_step. One pass instead of fsi_force_assemble + the collide
+ * phase. I16 storage: a measuring pass (store_pass 1) writes and counts
+ * nothing. */
+int fsi_lbm_collide_assembled(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const fsi_force_assembly* a,
+                              const void* pop_in, void* pop_out, void* cforce, void* ib_force, void* ib_total,
+                              int32_t* counters, void* stream);
+
 /* ------------------------------------------------------- reductions ---- */
 
 /* doubles per workgroup row of a reduction workspace */
@@ -408,8 +422,9 @@ int fsi_reduce_particles(const fsi_mpm_params* p, const fsi_mpm_buffers* b, cons
 #define FSI_MLP_OUT 3
 #define FSI_MLP_PARAMS 7312
 /* normalisation block of fsi_mlp_infer: mean[18], inv_std[18], lo[18],
- * hi[18] (envelope, normalised units), out_mean[3], out_std[3] */
-#define FSI_MLP_NORM 78
+ * hi[18] (envelope, normalised units), out_mean[3], out_std[3], y_lo[3],
+ * y_hi[3] (bounds of the normalised output: the training targets' range) */
+#define FSI_MLP_NORM 84
 
 /* One training batch: forward, loss = mean over rows and the 3 outputs of
  * (y - T)^2, backward. Rows idx[0..n) of X (n x 18 normalised features,

@@ -398,9 +398,10 @@ __global__ __launch_bounds__(kBlock) void k_mlp_adam(float* __restrict__ P, cons
 }
 
 // Inference: features X (raw, n x 18) -> correction out[a * stride + row]
-// = mean_y[a] + std_y[a] * y[a], or 0 with the row counted in *fallbacks
-// when a normalised feature leaves the training envelope. norm: mean[18],
-// inv_std[18], lo[18], hi[18], out_mean[3], out_std[3].
+// = mean_y[a] + std_y[a] * clamp(y[a], y_lo[a], y_hi[a]), or 0 with the row
+// counted in *fallbacks when a normalised feature leaves the training
+// envelope. norm: mean[18], inv_std[18], lo[18], hi[18], out_mean[3],
+// out_std[3], y_lo[3], y_hi[3] (the training targets' range, normalised).
 __global__ __launch_bounds__(kBlock) void k_mlp_infer(int n, const float* __restrict__ X, const float* __restrict__ P,
                                                     const float* __restrict__ norm, float* __restrict__ out,
                                                     int64_t stride, int32_t* __restrict__ fallbacks) {
@@ -425,7 +426,10 @@ __global__ __launch_bounds__(kBlock) void k_mlp_infer(int n, const float* __rest
     if (g == 0 && valid) {
 #pragma unroll
       for (int i = 0; i < kOut; ++i)
-        out[i * stride + r] = outside ? 0.0f : norm[4 * kIn + i] + norm[4 * kIn + kOut + i] * f.y[i];
+        out[i * stride + r] =
+            outside ? 0.0f
+                    : norm[4 * kIn + i] + norm[4 * kIn + kOut + i] * fminf(fmaxf(f.y[i], norm[4 * kIn + 2 * kOut + i]),
+                                                                           norm[4 * kIn + 3 * kOut + i]);
       nfall += outside ? 1 : 0;
     }
   }

@@ -361,6 +361,26 @@ __device__ __forceinline__ void x_face_boundary(R d[19], int mode, R rho_bc, con
   }
 }
 
+// Force assembly of the coupled step (fsi/coupling.py:408-461: phi clamp,
+// reset_force, coupling force, immersed-boundary drag) evaluated by the
+// collide pass that uses the force (ASM), instead of a pass of its own.
+template <typename R> struct AsmArgs {
+  R* cforce;         // 4N AoS coupling buffer (force density xyz, phi)
+  R* ib_force;       // 3N SoA IB drag (ib)
+  R* ib_total;       // 3, summed with atomics (ib)
+  int32_t* counters; // [5] active IB cells, [6] clipped (ib)
+  R* force_out;      // 3N SoA: the assembled lattice force
+  int coupling, clamp, ib, ib_use_max;
+  R drag, thr, max_force;
+  R base[3];
+};
+
+template <typename R> __device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 // One lattice update of one cell in the given phase (see the file comment).
 // FACES: some x face is not periodic. Requires 19 * N * sizeof(S) < 2^32.
 // Workgroup size of the lattice step. A workgroup reads 19 contiguous runs
@@ -370,13 +390,14 @@ __device__ __forceinline__ void x_face_boundary(R d[19], int mode, R rho_bc, con
 template <typename S> struct PullBlock { static constexpr int value = 256; };
 template <> struct PullBlock<float> { static constexpr int value = 512; };
 
-template <typename S, int PHASE, bool FORCE_FIELD, bool FACES>
+template <typename S, int PHASE, bool FORCE_FIELD, bool FACES, bool ASM = false>
 __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
     LbmK<typename PopStore<S>::real> c, const S* __restrict__ pin, S* __restrict__ pout,
     const uint32_t* __restrict__ nbmask, const typename PopStore<S>::real* __restrict__ force,
     typename PopStore<S>::real* __restrict__ rho_io, typename PopStore<S>::real* __restrict__ vel_io,
     const typename PopStore<S>::real* __restrict__ fv_prev,
-    typename PopStore<S>::real* __restrict__ fv_next) {
+    typename PopStore<S>::real* __restrict__ fv_next, AsmArgs<typename PopStore<S>::real> asm_args) {
+  static_assert(!ASM || PHASE == PHASE_COLLIDE, "the force assembly runs in the collide phase");
   using R = typename PopStore<S>::real;
   // the arithmetic runs on deviations d = F - w rho0 for every storage: the
   // 16-bit storages keep them, the absolute FP32/FP64 populations subtract
@@ -459,7 +480,67 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   for (int q = 0; q < 19; ++q) d[q] = PopStore<S>::load(raw[q], fac.ld[wclass(q)]) - base[wclass(q)];
 
   R fo[3];
-  if constexpr (FORCE_FIELD) {
+  if constexpr (ASM) {
+    // the force of this collision, assembled from the coupled step's
+    // buffers (fsi/coupling.py:408-461), written to the lattice force
+    const AsmArgs<R>& a = asm_args;
+    R* cf = a.cforce + 4 * (int64_t)n;
+    R phi = cf[3];
+    if (a.clamp && phi > R(1)) {
+      phi = R(1);
+      if (write) cf[3] = phi;
+    }
+    const R cx = cf[0], cy = cf[1], cz = cf[2];
+    fo[0] = solid ? R(0) : a.base[0];
+    fo[1] = solid ? R(0) : a.base[1];
+    fo[2] = solid ? R(0) : a.base[2];
+    if (a.coupling && !solid) {
+      fo[0] += cx;
+      fo[1] += cy;
+      fo[2] += cz;
+    }
+    if (a.ib) {
+      R fi[3] = {R(0), R(0), R(0)};
+      bool active = false, clipped = false;
+      if (a.drag > R(0) && !solid && phi > R(0) && phi >= a.thr) {
+        const R sc = -a.drag * phi * rho_io[n];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fi[k] = sc * vel_io[k * N + n];
+        const R nrm = sqrt(fi[0] * fi[0] + fi[1] * fi[1] + fi[2] * fi[2]);
+        if (a.ib_use_max && nrm > a.max_force && nrm > R(0)) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) fi[k] = fi[k] / nrm * a.max_force;
+          clipped = true;
+        }
+        active = true;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (write) a.ib_force[k * N + n] = fi[k];
+        fo[k] += fi[k];
+      }
+      if (!ADAPT || c.pass != 1) {  // uniform: a measuring pass counts nothing
+        active = active && live;
+        clipped = clipped && live;
+        const unsigned long long ab = __ballot(active), cb = __ballot(clipped), lanes = __ballot(1);
+        const int leader = __ffsll((long long)lanes) - 1;
+        if (ab) {
+          const R s0 = wave_sum(fi[0]), s1 = wave_sum(fi[1]), s2 = wave_sum(fi[2]);
+          if ((int)__lane_id() == leader) {
+            atomicAdd(a.ib_total + 0, s0);
+            atomicAdd(a.ib_total + 1, s1);
+            atomicAdd(a.ib_total + 2, s2);
+            atomicAdd(a.counters + 5, (int)__popcll(ab));
+          }
+        }
+        if (cb && (int)__lane_id() == leader) atomicAdd(a.counters + 6, (int)__popcll(cb));
+      }
+    }
+    if (write) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) a.force_out[k * N + n] = fo[k];
+    }
+  } else if constexpr (FORCE_FIELD) {
     fo[0] = force[n];
     fo[1] = force[N + n];
     fo[2] = force[2 * N + n];
@@ -686,20 +767,20 @@ int lbm_init_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pa, void
   return 0;
 }
 
-template <typename S, int PHASE, bool FF>
+template <typename S, int PHASE, bool FF, bool ASM = false>
 void launch_pull(const LbmK<typename PopStore<S>::real>& c, bool faces, const void* pin, void* pout,
                  const fsi_lbm_buffers* b, const void* force, const void* fv_prev, void* fv_next,
-                 hipStream_t st) {
+                 hipStream_t st, AsmArgs<typename PopStore<S>::real> a = {}) {
   using R = typename PopStore<S>::real;
   const dim3 grid(blocks_for(c.N, PullBlock<S>::value)), block(PullBlock<S>::value);
   if (faces)
-    hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, true>), grid, block, 0, st, c, (const S*)pin,
+    hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, true, ASM>), grid, block, 0, st, c, (const S*)pin,
                        (S*)pout, b->nbmask, (const R*)force, (R*)b->rho, (R*)b->vel,
-                       (const R*)fv_prev, (R*)fv_next);
+                       (const R*)fv_prev, (R*)fv_next, a);
   else
-    hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, false>), grid, block, 0, st, c, (const S*)pin,
+    hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, false, ASM>), grid, block, 0, st, c, (const S*)pin,
                        (S*)pout, b->nbmask, (const R*)force, (R*)b->rho, (R*)b->vel,
-                       (const R*)fv_prev, (R*)fv_next);
+                       (const R*)fv_prev, (R*)fv_next, a);
 }
 
 template <typename S>
@@ -733,6 +814,44 @@ int lbm_step_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* pi
   }
 #undef FSI_PULL
   FSI_TRY_LAUNCH("lbm_step");
+  return 0;
+}
+
+template <typename R> __global__ void k_reset_ib_sums(R* __restrict__ total, int32_t* __restrict__ counters) {
+  if (threadIdx.x < 3) total[threadIdx.x] = R(0);
+  if (threadIdx.x == 3) counters[5] = 0;
+  if (threadIdx.x == 4) counters[6] = 0;
+}
+
+template <typename S>
+int lbm_collide_assembled_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const fsi_force_assembly* fa,
+                            const void* pin, void* pout, void* cforce, void* ib_force, void* ib_total,
+                            int32_t* counters, hipStream_t st) {
+  using R = typename PopStore<S>::real;
+  LbmK<R> c = make_consts<R>(p, nullptr, b);
+  FSI_REQUIRE(19.0 * (double)c.N * sizeof(S) < 4294967296.0,
+              "lbm_collide_assembled: lattice of %lld cells too large for 32-bit plane offsets", (long long)c.N);
+  AsmArgs<R> a{};
+  a.cforce = (R*)cforce;
+  a.ib_force = (R*)ib_force;
+  a.ib_total = (R*)ib_total;
+  a.counters = counters;
+  a.force_out = (R*)b->force;
+  a.coupling = fa->coupling;
+  a.clamp = fa->clamp_fraction;
+  a.ib = fa->ib_enabled;
+  a.ib_use_max = fa->ib_use_max;
+  a.drag = R(fa->ib_drag);
+  a.thr = R(fa->ib_threshold);
+  a.max_force = R(fa->ib_max_force);
+  for (int k = 0; k < 3; ++k) a.base[k] = R(fa->base_force[k]);
+  if (a.ib && p->store_pass != 1) {
+    hipLaunchKernelGGL(k_reset_ib_sums<R>, dim3(1), dim3(64), 0, st, (R*)ib_total, counters);
+    FSI_TRY_LAUNCH("lbm_collide_assembled reset");
+  }
+  const bool faces = p->left_mode != FSI_BC_PERIODIC || p->right_mode != FSI_BC_PERIODIC;
+  launch_pull<S, PHASE_COLLIDE, false, true>(c, faces, pin, pout, b, nullptr, nullptr, nullptr, st, a);
+  FSI_TRY_LAUNCH("lbm_collide_assembled");
   return 0;
 }
 
@@ -782,6 +901,29 @@ int fsi_lbm_step(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const void* 
     case FSI_POP_F16: return lbm_step_t<__half>(p, b, pop_in, pop_out, force, phase, face_slot, st);
     case FSI_POP_I16: return lbm_step_t<int16_t>(p, b, pop_in, pop_out, force, phase, face_slot, st);
     default: return lbm_step_t<double>(p, b, pop_in, pop_out, force, phase, face_slot, st);
+  }
+}
+
+int fsi_lbm_collide_assembled(const fsi_lbm_params* p, const fsi_lbm_buffers* b, const fsi_force_assembly* a,
+                              const void* pop_in, void* pop_out, void* cforce, void* ib_force, void* ib_total,
+                              int32_t* counters, void* stream) {
+  if (int rc = check_params(p)) return rc;
+  FSI_REQUIRE(b && a && b->nbmask && b->rho && b->vel && b->force && cforce && pop_in && pop_out &&
+                  pop_in != pop_out,
+              "lbm_collide_assembled: null or aliased buffer");
+  FSI_REQUIRE(!a->ib_enabled || (ib_force && ib_total && counters),
+              "lbm_collide_assembled: IB needs ib_force, ib_total and counters");
+  FSI_REQUIRE(p->storage != FSI_POP_I16 || b->store_ctrl, "lbm_collide_assembled: I16 storage needs store_ctrl");
+  hipStream_t st = (hipStream_t)stream;
+  switch (p->storage) {
+    case FSI_POP_F32:
+      return lbm_collide_assembled_t<float>(p, b, a, pop_in, pop_out, cforce, ib_force, ib_total, counters, st);
+    case FSI_POP_F16:
+      return lbm_collide_assembled_t<__half>(p, b, a, pop_in, pop_out, cforce, ib_force, ib_total, counters, st);
+    case FSI_POP_I16:
+      return lbm_collide_assembled_t<int16_t>(p, b, a, pop_in, pop_out, cforce, ib_force, ib_total, counters, st);
+    default:
+      return lbm_collide_assembled_t<double>(p, b, a, pop_in, pop_out, cforce, ib_force, ib_total, counters, st);
   }
 }
 

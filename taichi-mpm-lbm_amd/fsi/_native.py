@@ -14,7 +14,7 @@ import torch
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libfsi_hip.so"
 # include/fsi_hip.h FSI_ABI_VERSION the structures below follow
-ABI_VERSION = 3
+ABI_VERSION = 4
 _lib = None
 
 POP_F32, POP_F16, POP_F64, POP_I16 = 0, 1, 2, 3
@@ -125,6 +125,7 @@ _SIGNATURES = {
     "fsi_contact": [_vp, _vp, _vp, _vp, _vp],
     "fsi_polar_rotation": [ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp],
     "fsi_force_assemble": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int32, _vp],
+    "fsi_lbm_collide_assembled": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "fsi_mpm_sort": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int32, _vp, _vp],
     "fsi_reduce_lattice": [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
     "fsi_reduce_particles": [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp],
@@ -139,7 +140,7 @@ _SIGNATURES = {
 }
 
 # learned-closure MLP shapes (FSI_MLP_* of the header)
-MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS, MLP_NORM = 18, 64, 3, 7312, 78
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS, MLP_NORM = 18, 64, 3, 7312, 84
 
 
 def library_path() -> Path:

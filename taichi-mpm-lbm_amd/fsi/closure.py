@@ -120,7 +120,9 @@ def _rows(t, ncols, device) -> torch.Tensor:
 
 @dataclasses.dataclass
 class Normalization:
-    """Feature/target standardisation and the training envelope."""
+    """Feature/target standardisation, the training envelope of the features
+    and the range of the (normalised) training targets, which bounds what
+    inference returns."""
 
     mean: np.ndarray
     std: np.ndarray
@@ -128,6 +130,8 @@ class Normalization:
     hi: np.ndarray
     out_mean: np.ndarray
     out_std: np.ndarray
+    out_lo: np.ndarray
+    out_hi: np.ndarray
 
     @classmethod
     def fit(cls, X: np.ndarray, T: np.ndarray, margin: float = 0.1) -> "Normalization":
@@ -140,12 +144,14 @@ class Normalization:
         pad = margin * (hi - lo)
         omean, ostd = T.mean(0), T.std(0)
         ostd = np.where(ostd > 1e-20, ostd, 1.0)
-        return cls(mean, std, lo - pad, hi + pad, omean, ostd)
+        Y = (T - omean) / ostd
+        return cls(mean, std, lo - pad, hi + pad, omean, ostd, Y.min(0), Y.max(0))
 
     def packed(self) -> np.ndarray:
-        """The kernels' 78-float block: mean, 1/std, lo, hi, out mean, out std."""
+        """The kernels' 84-float block: mean, 1/std, lo, hi, out mean, out
+        std, out lo, out hi."""
         return np.concatenate([self.mean, 1.0 / self.std, self.lo, self.hi, self.out_mean,
-                               self.out_std]).astype(np.float32)
+                               self.out_std, self.out_lo, self.out_hi]).astype(np.float32)
 
     def inputs(self, X) -> np.ndarray:
         return ((np.asarray(X, np.float64) - self.mean) / self.std).astype(np.float32)
@@ -208,13 +214,14 @@ class LearnedClosure:
         path.parent.mkdir(parents=True, exist_ok=True)
         n = self.norm
         np.savez(path, params=self.mlp.parameters_numpy(), mean=n.mean, std=n.std, lo=n.lo, hi=n.hi,
-                 out_mean=n.out_mean, out_std=n.out_std)
+                 out_mean=n.out_mean, out_std=n.out_std, out_lo=n.out_lo, out_hi=n.out_hi)
         return path
 
     @classmethod
     def load(cls, path) -> "LearnedClosure":
         with np.load(Path(path), allow_pickle=False) as d:
-            norm = Normalization(d["mean"], d["std"], d["lo"], d["hi"], d["out_mean"], d["out_std"])
+            norm = Normalization(d["mean"], d["std"], d["lo"], d["hi"], d["out_mean"], d["out_std"], d["out_lo"],
+                                 d["out_hi"])
             return cls(ClosureMLP(d["params"]), norm)
 
 
@@ -252,6 +259,18 @@ def refine_particles(parents: np.ndarray) -> np.ndarray:
     off = np.array([[sx, sy, sz] for sx in (-0.25, 0.25) for sy in (-0.25, 0.25) for sz in (-0.25, 0.25)])
     p = np.asarray(parents, np.float64)
     return (2.0 * p[:, None, :] + off[None, :, :]).reshape(-1, 3).astype(np.float32)
+
+
+def stable_substeps(youngs_modulus: float, poisson_ratio: float, density: float, courant: float = 0.3,
+                    minimum: int = 112) -> int:
+    """MPM substeps per lattice step (lattice dt = dx = 1) that keep the
+    elastic P-wave Courant number c_p dt / dx <= `courant`, c_p =
+    sqrt((lambda + 2 mu) / density), and at least `minimum`. The refined run
+    of a pair keeps c_p in cells per lattice step (acoustic scaling), so it
+    takes the same count per (fine) lattice step."""
+    E, nu = float(youngs_modulus), float(poisson_ratio)
+    p_modulus = E * (1.0 - nu) / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return max(int(minimum), int(math.ceil(math.sqrt(p_modulus / float(density)) / courant)))
 
 
 def pair_config(shape, refined: bool, n_particles: int, youngs_modulus: float, poisson_ratio: float,
@@ -310,19 +329,24 @@ class ClosureScene:
 
     @classmethod
     def random(cls, seed: int, shape=(48, 24, 24), axes=(3.0, 5.0), modulus=(2.0e2, 2.0e3),
-               poisson=(0.2, 0.4), inflow=(0.01, 0.05)) -> "ClosureScene":
+               poisson=(0.2, 0.4), inflow=(0.01, 0.05), substeps: int | str = 112) -> "ClosureScene":
         """Semi-axes U[axes], random orientation, E log-uniform in `modulus`,
-        nu U[poisson], inflow U[inflow]; centred a third along the channel."""
+        nu U[poisson], inflow U[inflow]; centred a third along the channel.
+        substeps="stable": the count ``stable_substeps`` gives the scene's
+        material."""
         from .scenes import random_rotation
 
         rng = np.random.default_rng(seed)
         semi = tuple(float(a) for a in rng.uniform(axes[0], axes[1], 3))
         R = random_rotation(rng)
         nx, ny, nz = shape
+        E = float(math.exp(rng.uniform(math.log(modulus[0]), math.log(modulus[1]))))
+        nu = float(rng.uniform(*poisson))
+        inflow_x = float(rng.uniform(*inflow))
+        n_sub = stable_substeps(E, nu, cls.density) if substeps == "stable" else int(substeps)
         return cls(shape=tuple(shape), center=(nx / 3.0, ny / 2.0, nz / 2.0), semi_axes=semi,
-                   rotation=tuple(float(v) for v in R.reshape(-1)),
-                   youngs_modulus=float(math.exp(rng.uniform(math.log(modulus[0]), math.log(modulus[1])))),
-                   poisson_ratio=float(rng.uniform(*poisson)), inflow=float(rng.uniform(*inflow)), seed=seed)
+                   rotation=tuple(float(v) for v in R.reshape(-1)), youngs_modulus=E, poisson_ratio=nu,
+                   inflow=inflow_x, substeps=n_sub, seed=seed)
 
     def particles(self) -> tuple[np.ndarray, np.ndarray]:
         """Coarse particles (spacing 0.5, jittered) inside the ellipsoid and
@@ -425,6 +449,9 @@ class PackScene:
     gap: float = 1.5
     upstream: float = 1.0
     seed: int = 0
+    # per-sphere Young's modulus, log-uniform in this range (a per-particle
+    # material plane; the config's global modulus is then unused), or None
+    body_modulus: tuple[float, float] | None = None
 
     @classmethod
     def random(cls, seed: int, shape=(48, 24, 24), count=4, radii=(2.5, 5.0), modulus=(2.0e2, 2.0e3),
@@ -450,14 +477,29 @@ class PackScene:
         ends = np.cumsum(counts)
         return [(int(e - c), int(e)) for c, e in zip(counts, ends)]
 
+    def body_moduli(self) -> np.ndarray | None:
+        """Young's modulus of each sphere (``body_modulus``: log-uniform,
+        numpy default_rng(seed + 7)), or None."""
+        if self.body_modulus is None:
+            return None
+        from .scenes import body_moduli
+
+        return body_moduli(self._pack()[1]["counts"], self.body_modulus[0], self.body_modulus[1],
+                           seed=self.seed + 7)[1]
+
     def config(self, refined: bool = False, storage: str = "fp32"):
         npart = len(self.particles()[1 if refined else 0])
-        return pair_config(self.shape, refined, npart, self.youngs_modulus, self.poisson_ratio, self.inflow,
+        E = self.youngs_modulus if self.body_modulus is None else float(self.body_moduli().max())
+        return pair_config(self.shape, refined, npart, E, self.poisson_ratio, self.inflow,
                            self.viscosity, self.gamma, self.substeps, self.density, storage)
 
     def simulation(self, refined: bool = False, storage: str = "fp32"):
         parents, children = self.particles()
-        return pair_simulation(self.config(refined, storage), children if refined else parents)
+        sim = pair_simulation(self.config(refined, storage), children if refined else parents)
+        if self.body_modulus is not None:
+            per_particle = np.repeat(self.body_moduli(), np.asarray(self._pack()[1]["counts"], np.int64))
+            sim.mpm.set_particle_material(np.repeat(per_particle, 8) if refined else per_particle)
+        return sim
 
 
 @functools.lru_cache(maxsize=8)
@@ -469,7 +511,8 @@ def _packed_spheres(scene: PackScene):
                        margin=scene.margin, gap=scene.gap)
 
 
-def run_observables(sim, steps: int, bodies, refined: bool = False, force_every: int = 0) -> dict:
+def run_observables(sim, steps: int, bodies, refined: bool = False, force_every: int = 0,
+                    progress: str | None = None) -> dict:
     """Runs `steps` coarse steps (twice as many fine steps when `refined`)
     and returns, in coarse lattice units, the drag per coarse step, each
     body's centroid displacement and, every `force_every` coarse steps, the
@@ -486,8 +529,13 @@ def run_observables(sim, steps: int, bodies, refined: bool = False, force_every:
     resolutions. A fine force is a quarter of the same force in coarse
     units; the fine drag of a coarse step is the mean of its two fine steps.
     `bodies` are coarse particle ranges (children of coarse particle i are
-    fine particles 8 i .. 8 i + 7)."""
+    fine particles 8 i .. 8 i + 7). `progress`: a label; a line on stderr
+    every 100 coarse steps."""
+    import sys
+    import time
+
     f = 2 if refined else 1
+    t0 = time.perf_counter()
     rows = [(f ** 3 * a, f ** 3 * b) for a, b in bodies]
     if force_every:
         sim.coupler.record_step_force()
@@ -508,6 +556,9 @@ def run_observables(sim, steps: int, bodies, refined: bool = False, force_every:
             if take:
                 pf = pf + sim.coupler.step_force_numpy().astype(np.float64)
         drag[k] /= f * f * f
+        if progress and (k + 1) % 100 == 0:
+            print(f"[{progress}] {k + 1}/{steps} coarse steps, {time.perf_counter() - t0:.0f} s", file=sys.stderr,
+                  flush=True)
         if take:
             forces.append(pf.reshape(-1, f ** 3, 3).sum(1) / (f * f * f) if refined else pf)
     return {"drag_x": drag, "displacement": centroids() - c0, "particle_forces": forces,
@@ -538,21 +589,23 @@ def observable_errors(run: dict, fine: dict) -> dict:
 
 
 def accuracy_vs_refined(scene, steps: int, closure: LearnedClosure | None = None, storage: str = "fp32",
-                        force_every: int = 5) -> dict:
+                        force_every: int = 5, progress: str | None = None) -> dict:
     """Drag, displacement and per-particle force errors of the coarse run,
     plain and with the closure (coarse populations stored as `storage`),
     against the refined FP32 run of the same scene (forces sampled every
     `force_every` coarse steps); the closure run's fallback count (particle
     evaluations outside the training envelope) and evaluations."""
     bodies = scene.bodies()
-    fine = run_observables(scene.simulation(True), steps, bodies, refined=True, force_every=force_every)
+    fine = run_observables(scene.simulation(True), steps, bodies, refined=True, force_every=force_every,
+                           progress=progress and progress + " refined")
     out = {"fine_mean_drag_x": float(fine["drag_x"].mean())}
     runs = [("plain", None)] + ([("closure", closure)] if closure is not None else [])
     for name, cl in runs:
         sim = scene.simulation(False, storage)
         if cl is not None:
             sim.coupler.set_closure(cl)
-        r = run_observables(sim, steps, bodies, force_every=force_every)
+        r = run_observables(sim, steps, bodies, force_every=force_every,
+                            progress=progress and f"{progress} coarse {name}")
         out[name] = observable_errors(r, fine)
         if cl is not None:
             out[name]["fallbacks"] = r["fallbacks"]

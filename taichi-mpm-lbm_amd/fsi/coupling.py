@@ -225,17 +225,19 @@ class LBMMpmCoupler:
         nat.call("fsi_mpm_substeps", nat.ref(mp), nat.ref(mb), nat.ref(cp), nat.ref(self._cbuf),
                  self.substeps, 1 if self.build_solid_volume_fraction else 0, 1)
 
-        # clamp phi, reset the lattice force, add the coupling force and IB drag
-        self.lbm._before_force_update()
+        # clamp phi, reset the lattice force, add the coupling force and IB
+        # drag, then the lattice step's collision with that force: one pass
         a = self._assembly(self.enabled, self.enabled and self.build_solid_volume_fraction)
-        nat.call("fsi_force_assemble", nat.ref(self.lbm._params), nat.ref(self.lbm._buffers),
-                 nat.ref(a), nat.ptr(self.cforce), nat.ptr(self.ib_force), nat.ptr(self.ib_total),
-                 nat.ptr(self.counters), 1)
-        uniform = None
-        if not self.enabled and not self.immersed_boundary_enabled:
-            uniform = self.lbm._rounded(self.lbm.config.force)
-        self.lbm._force_uniform = uniform
-        self.lbm.step()
+        if self.enabled or self.immersed_boundary_enabled:
+            self.lbm.collide_assembled(a, self.cforce, self.ib_force, self.ib_total, self.counters)
+        else:
+            # nothing couples: the force is the uniform base force
+            self.lbm._before_force_update()
+            nat.call("fsi_force_assemble", nat.ref(self.lbm._params), nat.ref(self.lbm._buffers),
+                     nat.ref(a), nat.ptr(self.cforce), nat.ptr(self.ib_force), nat.ptr(self.ib_total),
+                     nat.ptr(self.counters), 1)
+            self.lbm._force_uniform = self.lbm._rounded(self.lbm.config.force)
+            self.lbm.step()
         if self.enabled:
             self.coupled_step += 1
 

@@ -270,18 +270,25 @@ class LBMSolver3D:
     # store whatever force or boundary starts the flow
     _MEASURED_WRITES = 3
 
-    def _launch(self, phase: int) -> None:
-        f = self._force_arg()
+    def _launch(self, phase: int, assembly=None) -> None:
         self._params.store_epoch = self._epoch + 1
-        args = (nat.ref(self._params), nat.ref(self._buffers), nat.ptr(self.pop[self._cur]),
-                nat.ptr(self.pop[1 - self._cur]), nat.ref(f), phase, self._face_slot)
+        pin, pout = nat.ptr(self.pop[self._cur]), nat.ptr(self.pop[1 - self._cur])
+        if assembly is None:
+            name = "fsi_lbm_step"
+            f = self._force_arg()
+            args = (nat.ref(self._params), nat.ref(self._buffers), pin, pout, nat.ref(f), phase, self._face_slot)
+        else:  # the collide phase with the force assembled in the same pass
+            name = "fsi_lbm_collide_assembled"
+            a, cforce, ib_force, ib_total, counters = assembly
+            args = (nat.ref(self._params), nat.ref(self._buffers), nat.ref(a), pin, pout, nat.ptr(cforce),
+                    nat.ptr(ib_force), nat.ptr(ib_total), nat.ptr(counters))
         if self.store_ctrl is not None and self._epoch < self._MEASURED_WRITES:
             m = 3 * nat.STORE_CTRL_SLOT
             nat.fill_zero(self.store_ctrl[m:m + nat.STORE_CTRL_SLOT])
             self._params.store_pass = 1
-            nat.call("fsi_lbm_step", *args)
+            nat.call(name, *args)
             self._params.store_pass = 2
-        nat.call("fsi_lbm_step", *args)
+        nat.call(name, *args)
         self._params.store_pass = 0
         self._epoch += 1
         self._cur = 1 - self._cur
@@ -299,6 +306,21 @@ class LBMSolver3D:
         self._launch(nat.PHASE_COLLIDE if self._materialized else nat.PHASE_STREAM_COLLIDE)
         self._materialized = False
         self._force_dirty = False
+
+    def collide_assembled(self, assembly, cforce, ib_force, ib_total, counters) -> None:
+        """The coupled step's force assembly (fsi/coupling.py:162-170: phi
+        clamp, reset_force, coupling force, IB drag; ``assembly`` a
+        ForceAssembly) fused with the collision of the next lattice step
+        (lbm3d.py:314-333), which uses that force: one pass from the
+        materialised state. The lattice force field holds the assembled
+        force afterwards, as after fsi_force_assemble + step()."""
+        if not self._initialized:
+            raise RuntimeError("LBMSolver3D.initialize must be called before step().")
+        self._ensure_macro()
+        self._launch(nat.PHASE_COLLIDE, (assembly, cforce, ib_force, ib_total, counters))
+        self._materialized = False
+        self._force_dirty = False
+        self._force_uniform = None
 
     def _ensure_macro(self) -> None:
         """Materialise the reference's post-step state: f, rho, u (lbm3d.py:409-425)."""

@@ -11,6 +11,8 @@ transpose, block or bias gives O(1) errors. Features are computed in FP32:
 1e-5 of each feature's largest magnitude.
 """
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -113,6 +115,20 @@ def test_training_lowers_the_loss():
     far[:, 5] = 1e3
     assert np.all(closure.predict(far) == 0.0)
     assert int(closure.fallbacks.item()) - before == 4
+    # inside it, the correction stays within the training targets' range
+    n = closure.norm
+    assert np.allclose(n.out_lo, ((T - n.out_mean) / n.out_std).min(0)) and np.all(n.out_hi > n.out_lo)
+    lo, hi = n.out_mean + n.out_std * n.out_lo, n.out_mean + n.out_std * n.out_hi
+    near = Xt[:200] * 1.8  # inside the envelope, partly beyond the data
+    p = closure.predict(near)
+    assert np.all(p >= lo - 1e-5 * np.abs(lo)) and np.all(p <= hi + 1e-5 * np.abs(hi))
+    tight = dataclasses.replace(n, out_lo=np.full(mo.OUT, -0.05), out_hi=np.full(mo.OUT, 0.05))
+    from fsi.closure import LearnedClosure
+
+    clipped = LearnedClosure(closure.mlp, tight).predict(Xt)
+    lo, hi = n.out_mean - 0.05 * n.out_std, n.out_mean + 0.05 * n.out_std
+    assert np.all(clipped >= lo - 1e-5) and np.all(clipped <= hi + 1e-5)
+    assert np.isclose(clipped, lo, rtol=0, atol=1e-5).any() and np.isclose(clipped, hi, rtol=0, atol=1e-5).any()
 
 
 def test_features_match_oracle():
@@ -150,7 +166,7 @@ def test_zero_closure_keeps_the_penalty_force():
             p = mo.init_params(1)
             p[mo.O_W3:] = 0.0
             norm = Normalization(np.zeros(mo.IN), np.ones(mo.IN), np.full(mo.IN, -1e30), np.full(mo.IN, 1e30),
-                                 np.zeros(mo.OUT), np.ones(mo.OUT))
+                                 np.zeros(mo.OUT), np.ones(mo.OUT), np.full(mo.OUT, -1e30), np.full(mo.OUT, 1e30))
             sim.coupler.set_closure(LearnedClosure(ClosureMLP(p), norm))
         for _ in range(3):
             sim.step()
