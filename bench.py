@@ -514,15 +514,17 @@ def closure_data_and_training(n_scenes, steps=1500, every=50, warmup=50, adam_st
             "held_out": held, "closure_file": str(save_path) if save_path else None}
 
 
-def closure_large_scene(closure_path, steps=2000, speed_steps=2):
+def closure_large_scene(closure_path, steps=2000, speed_steps=2, plain_storages=("i16", "fp32")):
     """The large coupled scene of the closure's configuration: 64 elastic
     spheres (radius 5-14 refined cells, E log-uniform in [5e2, 5e3] per
     sphere, nu 0.3) in a 384x192x192 channel (FP32, plain penalty coupling),
     `steps` coarse steps = 2 x `steps` refined steps, against the 192x96x96
     run with FP16 populations, plain and with the closure saved at
-    `closure_path`: drag, centroid displacement, per-particle force errors,
-    fallback counts, and the time per coarse step of the closure run against
-    the refined run. Prints progress on stderr."""
+    `closure_path`, and plain with I16 and FP32 populations (storage or
+    resolution as the source of the coarse run's error): drag, centroid
+    displacement, per-particle force errors, fallback counts, and the time
+    per coarse step of the closure run against the refined run. Prints
+    progress on stderr."""
     import torch
 
     from fsi.closure import LearnedClosure, PackScene, accuracy_vs_refined, stable_substeps
@@ -531,7 +533,8 @@ def closure_large_scene(closure_path, steps=2000, speed_steps=2):
                       margin=4.0, gap=2.0, seed=5, body_modulus=(5.0e2, 5.0e3),
                       substeps=stable_substeps(5.0e3, 0.3, 1.2))
     closure = LearnedClosure.load(closure_path) if closure_path else None
-    acc = accuracy_vs_refined(scene, steps, closure, storage="fp16", force_every=50, progress="large scene")
+    acc = accuracy_vs_refined(scene, steps, closure, storage="fp16", force_every=50, progress="large scene",
+                              plain_storages=plain_storages)
     coarse = scene.simulation(False, "fp16")
     if closure is not None:
         coarse.coupler.set_closure(closure)

@@ -589,19 +589,22 @@ def observable_errors(run: dict, fine: dict) -> dict:
 
 
 def accuracy_vs_refined(scene, steps: int, closure: LearnedClosure | None = None, storage: str = "fp32",
-                        force_every: int = 5, progress: str | None = None) -> dict:
+                        force_every: int = 5, progress: str | None = None, plain_storages=()) -> dict:
     """Drag, displacement and per-particle force errors of the coarse run,
     plain and with the closure (coarse populations stored as `storage`),
     against the refined FP32 run of the same scene (forces sampled every
     `force_every` coarse steps); the closure run's fallback count (particle
-    evaluations outside the training envelope) and evaluations."""
+    evaluations outside the training envelope) and evaluations.
+    `plain_storages`: further plain coarse runs with other population
+    storages ("plain_<storage>")."""
     bodies = scene.bodies()
     fine = run_observables(scene.simulation(True), steps, bodies, refined=True, force_every=force_every,
                            progress=progress and progress + " refined")
     out = {"fine_mean_drag_x": float(fine["drag_x"].mean())}
-    runs = [("plain", None)] + ([("closure", closure)] if closure is not None else [])
-    for name, cl in runs:
-        sim = scene.simulation(False, storage)
+    runs = [("plain", None, storage)] + ([("closure", closure, storage)] if closure is not None else [])
+    runs += [(f"plain_{st}", None, st) for st in plain_storages]
+    for name, cl, st in runs:
+        sim = scene.simulation(False, st)
         if cl is not None:
             sim.coupler.set_closure(cl)
         r = run_observables(sim, steps, bodies, force_every=force_every,

@@ -132,7 +132,9 @@ def _vtk_array(values: np.ndarray, vector: bool):
 
 
 class SimulationOutputWriter:
-    """Write lightweight simulation snapshots for analysis and visualization."""
+    """Snapshot writer (reference: fsi/output.py): NPZ files of the lattice,
+    coupling and particle fields with the reference's keys, and VTK files
+    when pyevtk is importable, synchronously or from a worker thread."""
 
     def __init__(self, config: OutputConfig):
         self.config = config

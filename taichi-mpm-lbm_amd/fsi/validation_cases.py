@@ -90,9 +90,9 @@ def run_lbm_mass_conservation_case() -> ValidationReport:
         case_name="lbm_periodic_mass_conservation",
         metrics=(
             bounded_metric("relative_mass_error", relative_error(m1, m0), upper=1.0e-5,
-                           description="Relative total fluid mass drift on a periodic grid."),
+                           description="Fluid mass change over the run, relative to the start (all faces periodic)."),
             finite_metric("max_velocity_norm", s.max_velocity_norm(), upper=1.0e-4,
-                          description="Maximum velocity norm remains finite and near zero."),
+                          description="Largest |u| stays finite and close to rest."),
         ),
         metadata={"steps": steps, "grid": [s.nx, s.ny, s.nz]})
 
@@ -110,9 +110,9 @@ def run_lbm_force_response_case() -> ValidationReport:
         case_name="lbm_force_response",
         metrics=(
             bounded_metric("mean_ux_growth", ux1 - ux0, lower=1.0e-12,
-                           description="Mean x velocity grows under positive x force."),
+                           description="A +x body force raises the mean streamwise velocity."),
             finite_metric("max_velocity_norm", s.max_velocity_norm(), upper=1.0e-2,
-                          description="Forced velocity remains bounded in the smoke case."),
+                          description="Largest |u| of the forced lattice stays small."),
         ),
         metadata={"steps": steps, "grid": [s.nx, s.ny, s.nz]})
 
@@ -130,9 +130,9 @@ def run_mpm_zero_gravity_com_case() -> ValidationReport:
         case_name="mpm_zero_gravity_com_stability",
         metrics=(
             bounded_metric("center_of_mass_drift_norm", float(np.linalg.norm(c1 - c0)), upper=1.0e-3,
-                           units="grid", description="COM drift stays small without gravity or initial velocity."),
+                           units="grid", description="Centre of mass stays put when nothing drives the body."),
             finite_metric("max_velocity_norm", s.max_velocity_norm(), upper=1.0e-3,
-                          description="MPM velocities remain bounded without gravity."),
+                          description="Largest particle speed stays small with gravity off."),
         ),
         metadata={"steps": steps, "particle_count": s.particle_count()})
 
@@ -150,9 +150,9 @@ def run_mpm_gravity_response_case() -> ValidationReport:
         case_name="mpm_gravity_response",
         metrics=(
             bounded_metric("center_of_mass_y_delta", float(c1[1] - c0[1]), upper=-1.0e-12, units="grid",
-                           description="COM moves downward under weak negative y gravity."),
-            _finite_flag("positions_finite", s.positions_numpy(), "All particle positions remain finite."),
-            _finite_flag("velocities_finite", s.velocities_numpy(), "All particle velocities remain finite."),
+                           description="A small -y gravity moves the centre of mass down."),
+            _finite_flag("positions_finite", s.positions_numpy(), "No particle position is NaN or infinite."),
+            _finite_flag("velocities_finite", s.velocities_numpy(), "No particle velocity is NaN or infinite."),
         ),
         metadata={"steps": steps, "particle_count": s.particle_count()})
 
@@ -169,11 +169,11 @@ def run_coupled_drift_case() -> ValidationReport:
         case_name="coupled_enabled_vs_disabled_drift",
         metrics=(
             bounded_metric("enabled_dx", dx[True], lower=1.0e-7, units="grid",
-                           description="Enabled coupling advects the particle in +x."),
+                           description="With coupling on, the flow carries the particle along +x."),
             bounded_metric("disabled_dx_abs", abs(dx[False]), upper=1.0e-6, units="grid",
-                           description="Disabled coupling leaves the particle nearly fixed."),
+                           description="With coupling off, the particle barely moves."),
             bounded_metric("enabled_minus_disabled_dx", dx[True] - dx[False], lower=1.0e-7, units="grid",
-                           description="Enabled drift exceeds the disabled baseline."),
+                           description="The coupled particle travels farther than the uncoupled one."),
         ),
         metadata={"steps": 2, "gamma": 5.0, "initial_lbm_ux": 1.0e-3})
 
@@ -187,11 +187,11 @@ def run_coupling_force_balance_case() -> ValidationReport:
         case_name="coupling_force_balance",
         metrics=(
             bounded_metric("particle_force_x", float(particle[0]), lower=1.0e-12,
-                           description="Particle receives positive x force from uniform flow."),
+                           description="A +x stream pushes the particle along +x."),
             bounded_metric("fluid_force_x", float(fluid[0]), upper=-1.0e-12,
-                           description="Fluid receives opposite x reaction force."),
+                           description="The lattice gets the equal and opposite x force."),
             bounded_metric("force_balance_norm", norm, upper=1.0e-6,
-                           description="Particle force plus integrated fluid reaction is balanced."),
+                           description="Particle force and summed fluid reaction cancel."),
         ),
         metadata={"dt_ratio": 1.0, "cell_volume": sim.coupler.cell_volume})
 
@@ -213,11 +213,11 @@ def run_coupling_force_limit_case() -> ValidationReport:
         case_name="coupling_force_limit",
         metrics=(
             bounded_metric("particle_force_norm", float(np.linalg.norm(particle)), upper=0.05 + 1.0e-7,
-                           description="High-gamma particle coupling force is clipped."),
+                           description="A very stiff penalty rate hits the force limit."),
             bounded_metric("clipped_particle_count", float(d["clipped_particle_count"]), lower=1.0,
-                           description="At least one particle records force or velocity clipping."),
+                           description="The clipped-particle counter is non-zero."),
             bounded_metric("force_balance_norm", norm, upper=1.0e-6,
-                           description="Clipped particle force still balances fluid reaction."),
+                           description="After clipping, particle force and fluid reaction still cancel."),
         ),
         metadata={"gamma": 100.0, "force_limit": 0.05})
 
@@ -235,12 +235,12 @@ def run_coupling_boundary_support_case() -> ValidationReport:
         case_name="coupling_boundary_support",
         metrics=(
             bounded_metric("partial_support_particle_count", float(d["partial_support_particle_count"]),
-                           lower=1.0, description="Near-boundary particle records partial support."),
+                           lower=1.0, description="A particle next to the wall counts as partially supported."),
             bounded_metric("min_particle_valid_weight", float(d["min_particle_valid_weight"]), lower=1.0e-6,
-                           upper=1.0 - 1.0e-5, description="Valid support is finite and below full support."),
+                           upper=1.0 - 1.0e-5, description="Its fluid-node weight sum is finite and short of one."),
             bounded_metric("force_balance_norm", norm, upper=1.0e-6,
-                           description="Partial-support coupling still balances fluid reaction."),
-            _finite_flag("particle_force_finite", particle, "Boundary particle force remains finite."),
+                           description="With partial support, particle force and fluid reaction still cancel."),
+            _finite_flag("particle_force_finite", particle, "The wall-side particle's force is not NaN or infinite."),
         ),
         metadata={"particle_position": [-0.75, 4.0, 4.0], "min_valid_weight": 1.0e-6})
 
@@ -263,15 +263,15 @@ def run_immersed_boundary_drag_case() -> ValidationReport:
         case_name="immersed_boundary_drag",
         metrics=(
             bounded_metric("ib_active_cell_count", float(d["ib_active_cell_count"]), lower=1.0,
-                           description="At least one dynamic occupied fluid cell receives IB drag."),
+                           description="Some fluid cell covered by the moving body gets IB drag."),
             bounded_metric("ib_total_force_x", float(d["ib_total_force_x"]), upper=-1.0e-12,
-                           description="IB drag opposes positive x fluid velocity."),
+                           description="The IB drag points against the +x flow."),
             bounded_metric("ib_total_force_norm", float(d["ib_total_force_norm"]), lower=1.0e-12,
-                           description="IB drag total force is nonzero."),
+                           description="The summed IB drag is not zero."),
             bounded_metric("max_velocity_not_increased", float(sim.lbm.max_velocity_norm() - speed0),
-                           upper=1.0e-8, description="The IB smoke case does not increase max fluid speed."),
+                           upper=1.0e-8, description="IB drag does not raise the largest fluid speed."),
             _finite_flag("velocity_finite", sim.lbm.velocity_numpy(),
-                         "LBM velocity remains finite with IB drag enabled."),
+                         "With IB drag on, no lattice velocity is NaN or infinite."),
         ),
         metadata={"drag": 0.1, "particle_volume": 4.0})
 
@@ -296,13 +296,13 @@ def run_contact_diagnostics_case() -> ValidationReport:
         case_name="contact_diagnostics",
         metrics=(
             bounded_metric("contact_candidate_count", float(d["contact_candidate_count"]), lower=1.0,
-                           description="Static solid support marks at least one contact candidate."),
+                           description="A particle near the fixed solid is flagged for contact."),
             bounded_metric("contact_damped_particle_count", float(d["contact_damped_particle_count"]), lower=1.0,
-                           description="Contact damping is applied to at least one particle."),
+                           description="The contact damping counter is non-zero."),
             bounded_metric("particle_contact_mask_sum", float(sim.coupler.particle_contact_mask_numpy().sum()),
-                           lower=1.0, description="The active particle contact mask is set."),
+                           lower=1.0, description="The contact mask of the active particle is 1."),
             bounded_metric("particle_speed_delta", float(speed1 - speed0), upper=-1.0e-6,
-                           description="Contact damping reduces particle speed."),
+                           description="Contact damping slows the particle down."),
         ),
         metadata={"damping": 0.25, "initial_speed": speed0})
 
