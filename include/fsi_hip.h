@@ -236,6 +236,11 @@ typedef struct fsi_coupling_params {
   int32_t contact_enabled; /* run the contact pass after G2P */
   double contact_threshold;
   double contact_damping;
+  int32_t refine;          /* MPM cells per lattice cell along each axis: 1 (the reference:
+                              one co-located grid) or r > 1, an MPM grid r times finer than
+                              the lattice (MPM dims = r x lattice dims, lattice cell = r x
+                              MPM dx). The coupling and contact stencils then use the
+                              lattice's cells; cell_volume is the lattice cell's. 0 means 1 */
 } fsi_coupling_params;
 
 typedef struct fsi_coupling_buffers {

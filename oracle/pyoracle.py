@@ -422,7 +422,11 @@ class OracleCoupler:
 
     def __init__(self, config, lbm: OracleLBM, mpm: OracleMPM):
         config.validate()
-        if (lbm.nx, lbm.ny, lbm.nz) != (mpm.nx, mpm.ny, mpm.nz):
+        # MPM cells per lattice cell (1: the reference's co-located grids);
+        # the coupling and contact kernels take the lattice's dims and cell
+        # size, so a refined MPM grid needs nothing else
+        r = int(getattr(config, "mpm_refinement", 1))
+        if (r * lbm.nx, r * lbm.ny, r * lbm.nz) != (mpm.nx, mpm.ny, mpm.nz):
             raise ValueError("LBM and MPM grids must match.")
         self.config = config
         self.lbm, self.mpm = lbm, mpm
@@ -431,8 +435,8 @@ class OracleCoupler:
         self.enabled = bool(config.enabled)
         self.substeps = int(config.mpm_substeps_per_lbm_step)
         self.nx, self.ny, self.nz = lbm.nx, lbm.ny, lbm.nz
-        self.inv_dx = 1.0 / float(mpm.dx)
-        self.cell_volume = float(mpm.dx) ** 3
+        self.inv_dx = 1.0 / (r * float(mpm.dx))
+        self.cell_volume = (r * float(mpm.dx)) ** 3
         shape = (self.nx, self.ny, self.nz)
         dt = self.dtype
         self.coupling_force = np.zeros((*shape, 3), dt)

@@ -63,6 +63,11 @@ template <typename R> struct CplK {
   int contact;
   R contact_thr, contact_damp;
   R* force_mean;  // optional step-averaged coupling force accumulator (3 x cap)
+  // the lattice the coupling and contact stencils address: its dims and
+  // 1 / its cell size (the MPM grid's unless refine > 1)
+  int refine;
+  int lnx, lny, lnz;
+  R inv_dx_c;
 };
 
 template <typename R> CplK<R> make_cpl(const fsi_coupling_params* c) {
@@ -81,6 +86,19 @@ template <typename R> CplK<R> make_cpl(const fsi_coupling_params* c) {
   k.contact = c->contact_enabled;
   k.contact_thr = R(c->contact_threshold);
   k.contact_damp = R(c->contact_damping);
+  k.refine = c->refine > 1 ? c->refine : 1;
+  return k;
+}
+
+// CplK with the lattice geometry of an MPM grid p (MPM dims = refine x
+// lattice dims)
+template <typename R> CplK<R> make_cpl(const fsi_coupling_params* c, const fsi_mpm_params* p) {
+  CplK<R> k = make_cpl<R>(c);
+  k.refine = (c && c->refine > 1) ? c->refine : 1;
+  k.lnx = p->nx / k.refine;
+  k.lny = p->ny / k.refine;
+  k.lnz = p->nz / k.refine;
+  k.inv_dx_c = R(p->inv_dx / k.refine);
   return k;
 }
 
@@ -641,7 +659,10 @@ __device__ NodeBox workgroup_box(bool act, const int base[3], int (*red)[NW]) {
 // of fsi/coupling.py:332-406 fused in front when COUPLE). Lane particle p
 // (live: a particle of this workgroup). REGS: x, v and C come from the
 // caller's registers (the fused G2P+P2G launch) instead of memory.
-template <typename R, bool COUPLE, bool P2G, bool WIDE, bool REGS, int PL>
+// REF: the MPM grid is cp.refine times finer than the lattice; the coupling
+// interpolates and spreads on the lattice's cells (its own stencil, node box
+// and tile rows), the P2G on the MPM grid's.
+template <typename R, bool COUPLE, bool P2G, bool WIDE, bool REGS, int PL, bool REF = false>
 __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, int p, bool live, R* __restrict__ x,
                                          R* __restrict__ v, const R* __restrict__ C, R* __restrict__ Fd,
                                          const R* __restrict__ mass, const R* __restrict__ vol,
@@ -674,6 +695,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   // plane split (PL = 3): partial support and interpolated velocity
   __shared__ R cpart[PL > 1 && COUPLE ? PL : 1][PL > 1 && COUPLE ? kS : 1][4];
   static_assert(PL == 1 || (PL == 3 && sizeof(R) == 4), "plane split: FP32 only");
+  static_assert(!REF || PL == 1, "refined grids: one wavefront per particle");
   const int pln = PL > 1 ? (int)threadIdx.x / kS : 0;
   const int slot = PL > 1 ? (int)threadIdx.x % kS : (int)threadIdx.x;
   const bool writer = pln == 0;  // the wavefront that writes the particle
@@ -699,21 +721,45 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     locate(xp, m.inv_dx, base, fx);
   }
   bspline(fx, w);
+  // coupling stencil: the lattice's (REF), else the MPM stencil itself
+  int cbase[3] = {base[0], base[1], base[2]};
+  R cw[3][3];
+  if constexpr (REF) {
+    R cfx[3] = {R(0), R(0), R(0)};
+    if (act) locate(xp, cp.inv_dx_c, cbase, cfx);
+    bspline(cfx, cw);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) cw[a][b] = w[a][b];
+  }
+  const int lnx = REF ? cp.lnx : m.nx, lny = REF ? cp.lny : m.ny, lnz = REF ? cp.lnz : m.nz;
+  const int64_t NL = REF ? (int64_t)lnx * lny * lnz : N;  // lattice cells
 
   // bounding box of the workgroup's base cells -> tile of (box + 2) nodes
   constexpr int kBig = 1 << 30;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int mn[3], mx[3];
+  __shared__ int cred[REF ? 6 : 1][REF ? PL * kS / 64 : 1];
+  int mn[3], mx[3], cmn[3], cmx[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     mn[a] = wave_min(act ? base[a] : kBig);
     mx[a] = wave_max(act ? base[a] : -kBig);
+    if constexpr (REF) {
+      cmn[a] = wave_min(act ? cbase[a] : kBig);
+      cmx[a] = wave_max(act ? cbase[a] : -kBig);
+    }
   }
   if (lane == 0) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       red[a][wid] = mn[a];
       red[3 + a][wid] = mx[a];
+      if constexpr (REF) {
+        cred[a][wid] = cmn[a];
+        cred[3 + a][wid] = cmx[a];
+      }
     }
   }
   __syncthreads();
@@ -726,16 +772,34 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
       mn[a] = min(mn[a], red[a][wv]);
       mx[a] = max(mx[a], red[3 + a][wv]);
     }
+    if constexpr (REF) {
+      cmn[a] = cred[a][0];
+      cmx[a] = cred[3 + a][0];
+#pragma unroll
+      for (int wv = 1; wv < PL * kS / 64; ++wv) {
+        cmn[a] = min(cmn[a], cred[a][wv]);
+        cmx[a] = max(cmx[a], cred[3 + a][wv]);
+      }
+    } else {
+      cmn[a] = mn[a];
+      cmx[a] = mx[a];
+    }
   }
   const bool any = mn[0] <= mx[0];
   const long long tx = any ? (long long)mx[0] - mn[0] + 3 : 0;
   const long long ty = any ? (long long)mx[1] - mn[1] + 3 : 0;
   const long long tz = any ? (long long)mx[2] - mn[2] + 3 : 0;
   const long long tnodes = tx * ty * tz;
+  // the coupling's node box on the lattice (REF; else the same box)
+  const long long ctx = any ? (long long)cmx[0] - cmn[0] + 3 : 0;
+  const long long cty = any ? (long long)cmx[1] - cmn[1] + 3 : 0;
+  const long long ctz = any ? (long long)cmx[2] - cmn[2] + 3 : 0;
+  const long long ctnodes = ctx * cty * ctz;
   // scatter target of the workgroup: its LDS tile when the cells of its
   // particles fit, otherwise the global grid/coupling buffers directly
-  const bool use_lds = any && tnodes <= kN;
+  const bool use_lds = any && tnodes <= kN && (!REF || ctnodes <= kN);
   const int x0 = mn[0], y0 = mn[1], z0 = mn[2];
+  const int cx0 = cmn[0], cy0 = cmn[1], cz0 = cmn[2];
   if (boxes_out && threadIdx.x == 0) {
     // every node this workgroup writes (tile flush, untiled and deferred
     // scatters) lies in the box of its particles' stencils
@@ -753,6 +817,8 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   // scatter needs no per-node bounds test; out-of-lattice entries, and the
   // coupling values of solid nodes, are dropped at the flush
   const int tn_all = use_lds ? (int)tnodes : 0;
+  const int ctyi = use_lds ? (int)cty : 0, ctzi = use_lds ? (int)ctz : 0;
+  const int ctn_all = use_lds ? (int)ctnodes : 0;
 #define FSI_GLOBAL(node, slot, val)                      \
   do {                                                   \
     if ((slot) < 4)                                      \
@@ -760,17 +826,24 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     else                                                 \
       atomicAdd(cforce + 4 * (node) + ((slot) - 4), (val)); \
   } while (0)
-  for (int t = threadIdx.x; t < tn_all; t += blockDim.x) {
+  for (int t = threadIdx.x; t < (REF ? max(tn_all, ctn_all) : tn_all); t += blockDim.x) {
+    if (!REF || t < tn_all) {
 #pragma unroll
-    for (int s = 0; s < (COUPLE ? Tile::kValues : 4); ++s) tile[s * kN + t] = A(0);
+      for (int s = 0; s < (COUPLE && !REF ? Tile::kValues : 4); ++s) tile[s * kN + t] = A(0);
+    }
     if constexpr (COUPLE) {
+      if (REF && t >= ctn_all) continue;
+      if constexpr (REF) {
+#pragma unroll
+        for (int s = 4; s < Tile::kValues; ++s) tile[s * kN + t] = A(0);
+      }
       // stage the box's lattice velocity and fluid flag (in the lattice and
       // not solid) once per node; the 27-node gathers then read LDS
-      const int ni = x0 + t / (tzi * tyi), nj = y0 + (t / tzi) % tyi, nk = z0 + t % tzi;
+      const int ni = cx0 + t / (ctzi * ctyi), nj = cy0 + (t / ctzi) % ctyi, nk = cz0 + t % ctzi;
       bool in;
-      const int64_t node = gather_node(m.nx, m.ny, m.nz, ni, nj, nk, in);
+      const int64_t node = gather_node(lnx, lny, lnz, ni, nj, nk, in);
       const int32_t sd = solid[node];
-      const R u0 = lbm_vel[node], u1 = lbm_vel[N + node], u2 = lbm_vel[2 * N + node];
+      const R u0 = lbm_vel[node], u1 = lbm_vel[NL + node], u2 = lbm_vel[2 * NL + node];
       const bool ok = in && sd == 0;
       ctile[t][0] = ok ? u0 : R(0);
       ctile[t][1] = ok ? u1 : R(0);
@@ -813,6 +886,28 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     plane[0] = pln * tyz;
     wir[0] = P(pick3(pln, w[0][0], w[1][0], w[2][0]));
   }
+  // the same for the coupling's box and stencil (REF: on the lattice)
+  const int ctyz = REF ? ctyi * ctzi : tyz;
+  const int cb0 = REF ? ((cbase[0] - cx0) * ctyi + (cbase[1] - cy0)) * ctzi + (cbase[2] - cz0) : b0;
+  int crow[3], cplane[3];
+  P cwir[3], cwjr[3], cwkr[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    crow[s] = REF ? cb0 + jo[s] * ctzi : row[s];
+    cplane[s] = REF ? io[s] * ctyz : plane[s];
+  }
+  if constexpr (REF) {
+    rotate3<P>(ri, cw[0][0], cw[1][0], cw[2][0], cwir);
+    rotate3<P>(rj, cw[0][1], cw[1][1], cw[2][1], cwjr);
+    rotate3<P>(rk, cw[0][2], cw[1][2], cw[2][2], cwkr);
+  } else {
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      cwir[s] = wir[s];
+      cwjr[s] = wjr[s];
+      cwkr[s] = wkr[s];
+    }
+  }
 
   if constexpr (COUPLE) {
     if (live && !act && writer) {
@@ -828,14 +923,14 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
       // staged values are zero on invalid nodes and the flag is 0 or 1, so
       // the products are the masked sums without selects
       if constexpr (PL > 1) {
-        const R wi = pick3(pln, w[0][0], w[1][0], w[2][0]);
+        const R wi = pick3(pln, cw[0][0], cw[1][0], cw[2][0]);
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            const R* c = ctile[b0 + pln * tyz + j * tzi + k];
+            const R* c = ctile[cb0 + pln * ctyz + j * ctzi + k];
             const R c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
-            const R wt = wi * w[j][1] * w[k][2];
+            const R wt = wi * cw[j][1] * cw[k][2];
             wsum += wt * c3;
             u[0] += wt * c0;
             u[1] += wt * c1;
@@ -848,9 +943,9 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
           for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-              const R* c = ctile[b0 + i * tyz + j * tzi + k];
+              const R* c = ctile[cb0 + i * ctyz + j * ctzi + k];
               const R c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
-              const R wt = w[i][0] * w[j][1] * w[k][2];
+              const R wt = cw[i][0] * cw[j][1] * cw[k][2];
               wsum += wt * c3;
               u[0] += wt * c0;
               u[1] += wt * c1;
@@ -890,18 +985,18 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
               bool in;
-              const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i0 + ii, base[1] + j, base[2] + k, in);
+              const int64_t node = gather_node(lnx, lny, lnz, cbase[0] + i0 + ii, cbase[1] + j, cbase[2] + k, in);
               const int q = (ii * 3 + j) * 3 + k;
               sn[q] = solid[node];
               un[q][0] = lbm_vel[node];
-              un[q][1] = lbm_vel[N + node];
-              un[q][2] = lbm_vel[2 * N + node];
+              un[q][1] = lbm_vel[NL + node];
+              un[q][2] = lbm_vel[2 * NL + node];
               inmask |= (uint32_t)in << q;
             }
 #pragma unroll
         for (int ii = 0; ii < NB; ++ii) {
           const int i = i0 + ii;
-          const R wi = pick3(i, w[0][0], w[1][0], w[2][0]);
+          const R wi = pick3(i, cw[0][0], cw[1][0], cw[2][0]);
 #pragma unroll
           for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -909,7 +1004,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
               const int q = (ii * 3 + j) * 3 + k;
               const bool ok = ((inmask >> q) & 1u) && sn[q] == 0;
               valid |= (uint32_t)ok << (i * 9 + j * 3 + k);
-              const R wt = wi * w[j][1] * w[k][2];
+              const R wt = wi * cw[j][1] * cw[k][2];
               wsum += ok ? wt : R(0);
               u[0] += ok ? wt * un[q][0] : R(0);
               u[1] += ok ? wt * un[q][1] : R(0);
@@ -970,9 +1065,9 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
 #pragma unroll
               for (int k = 0; k < 3; ++k) {
                 if (!((valid >> ((i * 3 + j) * 3 + k)) & 1u)) continue;
-                const int ni = base[0] + i, nj = base[1] + j, nk = base[2] + k;
-                const R nw = (w[i][0] * w[j][1] * w[k][2]) * inv_wsum;
-                const int64_t node = ((int64_t)ni * m.ny + nj) * m.nz + nk;
+                const int ni = cbase[0] + i, nj = cbase[1] + j, nk = cbase[2] + k;
+                const R nw = (cw[i][0] * cw[j][1] * cw[k][2]) * inv_wsum;
+                const int64_t node = ((int64_t)ni * lny + nj) * lnz + nk;
                 FSI_GLOBAL(node, 4, nw * rf[0]);
                 FSI_GLOBAL(node, 5, nw * rf[1]);
                 FSI_GLOBAL(node, 6, nw * rf[2]);
@@ -1015,11 +1110,11 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
           for (int ii = 0; ii < kPlanes; ++ii)
 #pragma unroll
             for (int jj = 0; jj < 3; ++jj) {
-              const P wij = wir[ii] * wjr[jj];
-              A* tr = tile + row[jj] + plane[ii];
+              const P wij = cwir[ii] * cwjr[jj];
+              A* tr = tile + crow[jj] + cplane[ii];
 #pragma unroll
               for (int kk = 0; kk < 3; ++kk) {
-                const P wt = wij * wkr[kk];
+                const P wt = wij * cwkr[kk];
                 A* tn = tr + ko[kk];
 #pragma unroll
                 for (int k = 0; k < NV; ++k) TA::add(tn + off[k], wt * val[k]);
@@ -1172,6 +1267,19 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   // node; coupling entries only on fluid nodes
   if (use_lds) {
     __syncthreads();
+    if constexpr (REF && COUPLE) {
+      // the coupling values, on the lattice's box
+      for (int t = threadIdx.x; t < ctn_all; t += blockDim.x) {
+        const int ni = cx0 + t / ctyz, nj = cy0 + (t / ctzi) % ctyi, nk = cz0 + t % ctzi;
+        if (!inside(lnx, lny, lnz, ni, nj, nk) || ctile[t][3] == R(0)) continue;
+        const int64_t node = ((int64_t)ni * lny + nj) * lnz + nk;
+#pragma unroll
+        for (int s = 4; s < Tile::kValues; ++s) {
+          const A sum = tile[s * kN + t];
+          if (sum != A(0)) atomicAdd(cforce + 4 * node + (s - 4), R(TA::value(sum, cscale.inv[s - 4])));
+        }
+      }
+    }
     for (int t = threadIdx.x; t < tn_all; t += blockDim.x) {
       const int ni = x0 + t / tyz, nj = y0 + (t / tzi) % tyi, nk = z0 + t % tzi;
       if (!inside(m.nx, m.ny, m.nz, ni, nj, nk)) continue;
@@ -1181,7 +1289,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
         const A sum = tile[s * kN + t];
         if (sum != A(0)) atomicAdd(grid + 4 * node + s, R(TA::value(sum, pscale.inv[s])));
       }
-      if constexpr (COUPLE) {
+      if constexpr (COUPLE && !REF) {
         if (ctile[t][3] != R(0)) {
 #pragma unroll
           for (int s = 4; s < Tile::kValues; ++s) {
@@ -1233,7 +1341,7 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   }
 }
 
-template <typename R, bool COUPLE, bool P2G, bool WIDE, int PL>
+template <typename R, bool COUPLE, bool P2G, bool WIDE, int PL, bool REF = false>
 __global__ __launch_bounds__(PL * Slots<PL>::value) void k_p2g(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
                                                 R* __restrict__ v, const R* __restrict__ C,
                                                 R* __restrict__ Fd, const R* __restrict__ mass,
@@ -1268,7 +1376,7 @@ __global__ __launch_bounds__(PL * Slots<PL>::value) void k_p2g(MpmK<R> m, CplK<R
     p = blockIdx.x * kS + slot;
     live = p < m.num;
   }
-  p2g_pass<R, COUPLE, P2G, WIDE, false, PL>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid, solid, lbm_vel,
+  p2g_pass<R, COUPLE, P2G, WIDE, false, PL, REF>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid, solid, lbm_vel,
                                         cforce, valid_weight, cmask, counters, clear_force, boxes_out, big_flag,
                                         big_value, closure, nullptr, nullptr, nullptr);
 }
@@ -1295,15 +1403,17 @@ __device__ __forceinline__ void grid_velocity(const MpmK<R>& m, const R g[4], in
   }
 }
 
-// fsi/coupling.py:463-496 for one particle at position xp
-template <typename R>
+// fsi/coupling.py:463-496 for one particle at position xp (REF: on the
+// lattice's cells of a refined MPM grid)
+template <typename R, bool REF = false>
 __device__ __forceinline__ bool contact_test(const MpmK<R>& m, const CplK<R>& cp, const R xp[3],
                                              const int32_t* __restrict__ solid,
                                              const R* __restrict__ cforce) {
   int base[3];
   R fx[3], w[3][3];
-  locate(xp, m.inv_dx, base, fx);
+  locate(xp, REF ? cp.inv_dx_c : m.inv_dx, base, fx);
   bspline(fx, w);
+  const int lnx = REF ? cp.lnx : m.nx, lny = REF ? cp.lny : m.ny, lnz = REF ? cp.lnz : m.nz;
   R dyn = R(0), stat = R(0);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -1312,7 +1422,7 @@ __device__ __forceinline__ bool contact_test(const MpmK<R>& m, const CplK<R>& cp
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         bool in;
-        const int64_t node = gather_node(m.nx, m.ny, m.nz, base[0] + i, base[1] + j, base[2] + k, in);
+        const int64_t node = gather_node(lnx, lny, lnz, base[0] + i, base[1] + j, base[2] + k, in);
         const R phi = cforce[4 * node + 3];
         const int32_t sflag = solid[node];
         const R wt = w[i][0] * w[j][1] * w[k][2];
@@ -1362,7 +1472,7 @@ __device__ __forceinline__ void zero_grid_pass(const MpmK<R>& m, uint4* __restri
 // `plane` gathers the 9 nodes of x offset `plane`, the partial sums of v and
 // C are added in plane order through LDS, every plane continues with the
 // same totals and plane 0 alone writes the particle.
-template <typename R, bool CONTACT, bool WIDE, int PL, int LDS = g2p_lds_bytes<R>()>
+template <typename R, bool CONTACT, bool WIDE, int PL, int LDS = g2p_lds_bytes<R>(), bool REF = false>
 __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, int p, bool live, R* __restrict__ x,
                                          R* __restrict__ v, R* __restrict__ C, const int32_t* __restrict__ active,
                                          const R* __restrict__ grid, const int32_t* __restrict__ solid,
@@ -1536,7 +1646,7 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
       if (writer) x[a * cap + p] = xp[a];
     }
     if constexpr (CONTACT) {
-      cand = contact_test(m, cp, xp, solid, cforce);
+      cand = contact_test<R, REF>(m, cp, xp, solid, cforce);
       if (cand && cp.contact_damp > R(0)) {
         damped = true;
 #pragma unroll
@@ -1568,7 +1678,7 @@ __device__ __forceinline__ void g2p_pass(const MpmK<R>& m, const CplK<R>& cp, in
   }
 }
 
-template <typename R, bool CONTACT, bool WIDE, int PL>
+template <typename R, bool CONTACT, bool WIDE, int PL, bool REF = false>
 __global__ __launch_bounds__(PL * Slots<PL>::value) void k_g2p(MpmK<R> m, CplK<R> cp, R* __restrict__ x,
                                                 R* __restrict__ v, R* __restrict__ C,
                                                 const int32_t* __restrict__ active,
@@ -1602,7 +1712,7 @@ __global__ __launch_bounds__(PL * Slots<PL>::value) void k_g2p(MpmK<R> m, CplK<R
     live = p < m.num;
   }
   R xo[3], vo[3], Co[9];
-  g2p_pass<R, CONTACT, WIDE, PL>(m, cp, p, live, x, v, C, active, grid, solid, cforce, contact_mask, counters, xo, vo,
+  g2p_pass<R, CONTACT, WIDE, PL, g2p_lds_bytes<R>(), REF>(m, cp, p, live, x, v, C, active, grid, solid, cforce, contact_mask, counters, xo, vo,
                                  Co);
 }
 
@@ -1616,7 +1726,7 @@ __global__ __launch_bounds__(PL * Slots<PL>::value) void k_g2p(MpmK<R> m, CplK<R
 // workgroups per CU): measured 4.7% faster per coupled step on a 448,861
 // particle suspension than 146 VGPRs at 3, despite a 14-byte spill; the wide
 // variants would spill ~190 bytes and keep their registers.
-template <typename R, bool COUPLE, bool CONTACT, bool WIDE, int PL>
+template <typename R, bool COUPLE, bool CONTACT, bool WIDE, int PL, bool REF = false>
 __global__ __launch_bounds__(PL * Slots<PL>::value)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && PL == 1 && !WIDE ? 4 : 1))) void k_g2p_p2g(
     MpmK<R> m, CplK<R> cp, R* __restrict__ x, R* __restrict__ v, R* __restrict__ C, R* __restrict__ Fd,
@@ -1643,13 +1753,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && PL == 1 && !WIDE ? 4 : 1)))
     live = p < m.num;
   }
   R xo[3], vo[3], Co[9];
-  g2p_pass<R, CONTACT, WIDE, PL, pass_lds_bytes<R, COUPLE>()>(m, cp, p, live, x, v, C, active, grid_src, solid, cforce,
+  g2p_pass<R, CONTACT, WIDE, PL, pass_lds_bytes<R, COUPLE>(), REF>(m, cp, p, live, x, v, C, active, grid_src, solid, cforce,
                                                           contact_mask, g2p_counters, xo, vo, Co);
   // the P2G stage reuses the G2P stage's LDS buffer: every read of the
   // staged grid velocities completes before any thread writes its tile
   __syncthreads();
   if (has)  // uniform over the workgroup
-    p2g_pass<R, COUPLE, true, WIDE, true, PL>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid_dst, solid,
+    p2g_pass<R, COUPLE, true, WIDE, true, PL, REF>(m, cp, p, live, x, v, C, Fd, mass, vol, active, pforce, grid_dst, solid,
                                           lbm_vel, cforce, valid_weight, cmask, p2g_counters, clear_force, boxes_out,
                                           big_out, big_value, closure, xo, vo, Co);
 }
@@ -1742,7 +1852,7 @@ __global__ __launch_bounds__(kBlock) void k_closure_features(MpmK<R> m, const R*
   out[17] = (float)sqrt(rel[0] * rel[0] + rel[1] * rel[1] + rel[2] * rel[2]);
 }
 
-template <typename R>
+template <typename R, bool REF = false>
 __global__ __launch_bounds__(kBlock) void k_contact(MpmK<R> m, CplK<R> cp, const R* __restrict__ x,
                                                     R* __restrict__ v,
                                                     const int32_t* __restrict__ active,
@@ -1756,7 +1866,7 @@ __global__ __launch_bounds__(kBlock) void k_contact(MpmK<R> m, CplK<R> cp, const
   bool cand = false, damped = false;
   if (act) {
     const R xp[3] = {x[p], x[m.cap + p], x[2 * m.cap + p]};
-    cand = contact_test(m, cp, xp, solid, cforce);
+    cand = contact_test<R, REF>(m, cp, xp, solid, cforce);
     if (cand && cp.contact_damp > R(0)) {
       damped = true;
 #pragma unroll
@@ -1797,16 +1907,21 @@ int check_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
 // Plane split of the particle passes (three wavefronts per particle, one per
 // x plane of the stencil): FP32, a work list of at most split_max_chunks
 // chunks (one-batch gathers).
-template <typename R> bool plane_split(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
-  return sizeof(R) == 4 && b->chunks && p->num_chunks > 0 && p->num_chunks <= p->split_max_chunks &&
+// a refined MPM grid (coupling refine > 1) runs one wavefront per particle
+inline bool refined(const fsi_coupling_params* c) { return c && c->refine > 1; }
+
+template <typename R>
+bool plane_split(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c) {
+  return sizeof(R) == 4 && !refined(c) && b->chunks && p->num_chunks > 0 && p->num_chunks <= p->split_max_chunks &&
          (!b->zero_boxes || (int64_t)Slots<3>::per_chunk * p->num_chunks <= (int64_t)b->zero_boxes_cap);
 }
 
 // Particle workgroups of one P2G or G2P pass (one node box each).
-template <typename R> int64_t particle_workgroups(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
+template <typename R>
+int64_t particle_workgroups(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c) {
   if (p->num_particles == 0) return 0;
   if (b->chunks && p->num_chunks > 0)
-    return (int64_t)p->num_chunks * (plane_split<R>(p, b) ? Slots<3>::per_chunk : 1);
+    return (int64_t)p->num_chunks * (plane_split<R>(p, b, c) ? Slots<3>::per_chunk : 1);
   return blocks_for(p->num_particles);
 }
 
@@ -1817,7 +1932,7 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
           int32_t* big_flag = nullptr, int big_value = 0, int32_t* counters_override = nullptr) {
   MpmK<R> m = make_mpm<R>(p, b);
   int32_t* ctrs = counters_override ? counters_override : (cb ? cb->counters : nullptr);
-  CplK<R> cp = make_cpl<R>(c);
+  CplK<R> cp = make_cpl<R>(c, p);
   cp.force_mean = cb ? (R*)cb->force_mean : nullptr;
   const bool couple = c && c->enabled;
   if (couple && zero_counters) {
@@ -1827,7 +1942,7 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
   if (m.num == 0) return 0;
   // with a work list from the particle sort, one workgroup per chunk
   const int32_t* chunks = (b->chunks && p->num_chunks > 0) ? b->chunks : nullptr;
-  const dim3 grid((unsigned)particle_workgroups<R>(p, b)), block(kBlock);
+  const dim3 grid((unsigned)particle_workgroups<R>(p, b, c)), block(kBlock);
 #define FSI_P2G_ARGS                                                                            \
   m, cp, (R*)b->x, (R*)b->v, (const R*)b->C, (R*)b->F, (const R*)b->mass, (const R*)b->volume, \
       b->active, (R*)b->pforce, grid_target, cb ? cb->solid : nullptr,                          \
@@ -1839,7 +1954,8 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
   // gathers' registers) takes the one-batch gathers: its time is one
   // workgroup's latency
   const bool wide = (int64_t)grid.x <= (int64_t)p->wide_max_workgroups;
-  const bool split = plane_split<R>(p, b);
+  const bool split = plane_split<R>(p, b, c);
+  const bool ref = refined(c);
 #define FSI_P2G_LAUNCH(C, P)                                                                          \
   do {                                                                                                \
     if constexpr (sizeof(R) == 4) {                                                                   \
@@ -1848,7 +1964,12 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
         break;                                                                                        \
       }                                                                                               \
     }                                                                                                 \
-    if (wide)                                                                                         \
+    if (ref && C) {                                                                                   \
+      if (wide)                                                                                       \
+        hipLaunchKernelGGL((k_p2g<R, C, P, true, 1, true>), grid, block, 0, st, FSI_P2G_ARGS);        \
+      else                                                                                            \
+        hipLaunchKernelGGL((k_p2g<R, C, P, false, 1, true>), grid, block, 0, st, FSI_P2G_ARGS);       \
+    } else if (wide)                                                                                  \
       hipLaunchKernelGGL((k_p2g<R, C, P, true, 1>), grid, block, 0, st, FSI_P2G_ARGS);                \
     else                                                                                              \
       hipLaunchKernelGGL((k_p2g<R, C, P, false, 1>), grid, block, 0, st, FSI_P2G_ARGS);               \
@@ -1875,7 +1996,7 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
                int32_t* counters_override = nullptr) {
   MpmK<R> m = make_mpm<R>(p, b);
   int32_t* ctrs = counters_override ? counters_override : (cb ? cb->counters : nullptr);
-  CplK<R> cp = make_cpl<R>(c);
+  CplK<R> cp = make_cpl<R>(c, p);
   const bool contact = c && c->contact_enabled;
   if (contact && zero_counters) {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 3, 2);
@@ -1888,7 +2009,7 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   // with a work list from the particle sort, one workgroup per chunk (as P2G)
   const int32_t* chunks = (b->chunks && p->num_chunks > 0) ? b->chunks : nullptr;
   const int num_chunks = chunks ? p->num_chunks : 0;
-  const int64_t particle_blocks = particle_workgroups<R>(p, b);
+  const int64_t particle_blocks = particle_workgroups<R>(p, b, c);
   const int64_t blocks = std::max<int64_t>(particle_blocks, std::max<int64_t>(zero_blocks, zero_ctr ? 1 : 0));
   if (blocks == 0) return 0;
   // particle workgroups that fit on the chip at once take the one-batch
@@ -1902,7 +2023,7 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   const dim3 g2p_grid((unsigned)blocks), g2p_block(kBlock);
   bool launched = false;
   if constexpr (sizeof(R) == 4) {
-    if (plane_split<R>(p, b)) {
+    if (plane_split<R>(p, b, c)) {
       if (contact)
         hipLaunchKernelGGL((k_g2p<R, true, false, 3>), g2p_grid, dim3(3 * Slots<3>::value), 0, st,
                            FSI_G2P_ARGS(true));
@@ -1913,7 +2034,11 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
     }
   }
   if (launched) {
-  } else if (contact && wide)
+  } else if (contact && refined(c) && wide)
+    hipLaunchKernelGGL((k_g2p<R, true, true, 1, true>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
+  else if (contact && refined(c))
+    hipLaunchKernelGGL((k_g2p<R, true, false, 1, true>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
+  else if (contact && wide)
     hipLaunchKernelGGL((k_g2p<R, true, true, 1>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
   else if (contact)
     hipLaunchKernelGGL((k_g2p<R, true, false, 1>), g2p_grid, g2p_block, 0, st, FSI_G2P_ARGS(true));
@@ -1979,7 +2104,7 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
   local.build_fraction = 0;
   const fsi_coupling_params* lc = c ? &local : nullptr;
   MpmK<R> m = make_mpm<R>(p, b);
-  CplK<R> cp = make_cpl<R>(lc);
+  CplK<R> cp = make_cpl<R>(lc, p);
   cp.force_mean = cb ? (R*)cb->force_mean : nullptr;
   const bool couple = lc && lc->enabled;
   const bool contact = lc && lc->contact_enabled;
@@ -1988,7 +2113,8 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
   const int64_t zero_blocks = std::min<int64_t>(blocks_for(n16), 1024);
   const dim3 grid((unsigned)std::max<int64_t>(num_wg, zero_blocks)), block(kBlock);
   const bool wide = num_wg <= (int64_t)p->wide_max_workgroups;
-  const bool split = plane_split<R>(p, b);
+  const bool split = plane_split<R>(p, b, c);
+  const bool ref = refined(c);
   for (int s = 1; s < substeps; ++s) {
     const int q = (s + 1) % 3;
     const bool by_boxes = s >= 2;
@@ -2008,7 +2134,12 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
         break;                                                                                            \
       }                                                                                                   \
     }                                                                                                     \
-    if (wide)                                                                                             \
+    if (ref && (CPL || CON)) {                                                                            \
+      if (wide)                                                                                           \
+        hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, true, 1, true>), grid, block, 0, st, FSI_FUSED_ARGS);  \
+      else                                                                                                \
+        hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false, 1, true>), grid, block, 0, st, FSI_FUSED_ARGS); \
+    } else if (wide)                                                                                      \
       hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, true, 1>), grid, block, 0, st, FSI_FUSED_ARGS);          \
     else                                                                                                  \
       hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false, 1>), grid, block, 0, st, FSI_FUSED_ARGS);         \
@@ -2054,7 +2185,7 @@ int substeps_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   // dirty boxes: P2G pass s records its workgroups' boxes under parity s%2;
   // G2P pass s >= 1 zeroes the other grid from the boxes of pass s-1 (pass 0
   // zeroes it whole: its contents are unknown)
-  const int64_t num_wg = particle_workgroups<R>(p, b);
+  const int64_t num_wg = particle_workgroups<R>(p, b, c);
   const bool boxes = two && b->zero_boxes && num_wg <= (int64_t)b->zero_boxes_cap;
   // three grids, box scratch for three passes and counter rows: one launch
   // per substep boundary
@@ -2089,13 +2220,18 @@ template <typename R>
 int contact_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
               const fsi_coupling_buffers* cb, hipStream_t st) {
   MpmK<R> m = make_mpm<R>(p, b);
-  CplK<R> cp = make_cpl<R>(c);
+  CplK<R> cp = make_cpl<R>(c, p);
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cb->counters, 3, 2);
   FSI_TRY_LAUNCH("zero_counters");
   if (m.num == 0) return 0;
-  hipLaunchKernelGGL(k_contact<R>, dim3(blocks_for(m.num)), dim3(kBlock), 0, st, m, cp,
-                     (const R*)b->x, (R*)b->v, b->active, cb->solid, (const R*)cb->cforce,
-                     cb->contact_mask, cb->counters);
+  if (refined(c))
+    hipLaunchKernelGGL((k_contact<R, true>), dim3(blocks_for(m.num)), dim3(kBlock), 0, st, m, cp,
+                       (const R*)b->x, (R*)b->v, b->active, cb->solid, (const R*)cb->cforce,
+                       cb->contact_mask, cb->counters);
+  else
+    hipLaunchKernelGGL(k_contact<R>, dim3(blocks_for(m.num)), dim3(kBlock), 0, st, m, cp,
+                       (const R*)b->x, (R*)b->v, b->active, cb->solid, (const R*)cb->cforce,
+                       cb->contact_mask, cb->counters);
   FSI_TRY_LAUNCH("contact");
   return 0;
 }
@@ -2152,9 +2288,22 @@ int noalloc_scratch(int32_t** out) {
   return 0;
 }
 
+// a refined MPM grid: dims a multiple of the lattice's, no learned closure
+// (its features are sampled on the MPM stencil)
+int check_refine(const fsi_mpm_params* p, const fsi_coupling_params* c, const fsi_coupling_buffers* cb,
+                 const char* name) {
+  if (!refined(c)) return 0;
+  const int r = c->refine;
+  FSI_REQUIRE(p->nx % r == 0 && p->ny % r == 0 && p->nz % r == 0,
+              "%s: MPM grid %d x %d x %d is not %d x a lattice", name, p->nx, p->ny, p->nz, r);
+  FSI_REQUIRE(!cb || !cb->closure_force, "%s: the learned closure needs co-located grids (refine 1)", name);
+  return 0;
+}
+
 int check_substeps(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
                    const fsi_coupling_buffers* cb, int32_t substeps, const char* name) {
   if (int rc = check_mpm(p, b)) return rc;
+  if (int rc = check_refine(p, c, cb, name)) return rc;
   FSI_REQUIRE(substeps >= 0, "%s: negative substep count", name);
   FSI_REQUIRE(b->x && b->v && b->C && b->F && b->mass && b->volume && b->active && b->pforce && b->grid,
               "%s: null particle or grid buffer", name);
@@ -2186,6 +2335,7 @@ int fsi_mpm_clear_grid(const fsi_mpm_params* p, const fsi_mpm_buffers* b, void* 
 int fsi_mpm_p2g(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
                 const fsi_coupling_buffers* cb, int32_t clear_force, int32_t do_p2g, void* stream) {
   if (int rc = check_mpm(p, b)) return rc;
+  if (int rc = check_refine(p, c, cb, "mpm_p2g")) return rc;
   FSI_REQUIRE(b->x && b->v && b->C && b->F && b->mass && b->volume && b->active && b->pforce,
               "mpm_p2g: null particle buffer");
   FSI_REQUIRE(!do_p2g || b->grid, "mpm_p2g: null grid");
@@ -2202,6 +2352,7 @@ int fsi_mpm_p2g(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_cou
 int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
                      const fsi_coupling_params* c, const fsi_coupling_buffers* cb, void* stream) {
   if (int rc = check_mpm(p, b)) return rc;
+  if (int rc = check_refine(p, c, cb, "mpm_grid_g2p")) return rc;
   FSI_REQUIRE(b->x && b->v && b->C && b->active && b->grid, "mpm_grid_g2p: null buffer");
   if (c && c->contact_enabled)
     FSI_REQUIRE(cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters,
@@ -2215,6 +2366,7 @@ int fsi_mpm_grid_g2p(const fsi_mpm_params* p, const fsi_mpm_buffers* b,
 int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c,
                 const fsi_coupling_buffers* cb, void* stream) {
   if (int rc = check_mpm(p, b)) return rc;
+  if (int rc = check_refine(p, c, cb, "contact")) return rc;
   FSI_REQUIRE(c && cb && cb->solid && cb->cforce && cb->contact_mask && cb->counters && b->x &&
                   b->v && b->active,
               "contact: null buffer");
@@ -2238,7 +2390,7 @@ int fsi_mpm_substeps_noalloc(const fsi_mpm_params* p, const fsi_mpm_buffers* b, 
                              const fsi_coupling_buffers* cb, int32_t substeps, int32_t build_first,
                              int32_t clear_force, void* stream) {
   if (int rc = check_substeps(p, b, c, cb, substeps, "mpm_substeps_noalloc")) return rc;
-  const int64_t wg = p->real == FSI_REAL_F64 ? particle_workgroups<double>(p, b) : particle_workgroups<float>(p, b);
+  const int64_t wg = p->real == FSI_REAL_F64 ? particle_workgroups<double>(p, b, c) : particle_workgroups<float>(p, b, c);
   FSI_REQUIRE(wg <= kNoAllocMaxWorkgroups,
               "mpm_substeps_noalloc: %lld particle workgroups per pass exceed the static scratch's %d "
               "(use fsi_mpm_substeps with caller-provided zero_boxes)",

@@ -88,7 +88,7 @@ class CouplingParams(ctypes.Structure):
         ("use_relative_velocity_limit", ctypes.c_int32), ("relative_velocity_limit", ctypes.c_double),
         ("use_force_limit", ctypes.c_int32), ("force_limit", ctypes.c_double),
         ("contact_enabled", ctypes.c_int32), ("contact_threshold", ctypes.c_double),
-        ("contact_damping", ctypes.c_double),
+        ("contact_damping", ctypes.c_double), ("refine", ctypes.c_int32),
     ]
 
 

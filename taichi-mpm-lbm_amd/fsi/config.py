@@ -156,9 +156,17 @@ class CouplingConfig:
     contact_enabled: bool = False
     contact_velocity_damping: float = 0.0
     contact_fraction_threshold: float = 0.5
+    # MPM cells per lattice cell along each axis (no counterpart in the
+    # reference, whose grids are one, config.py:246-255): r > 1 couples an
+    # MPM grid r times finer than the lattice (MPM dims = r x LBM dims, one
+    # lattice cell = r MPM dx); the coupling and contact stencils then use
+    # the lattice's cells
+    mpm_refinement: int = 1
 
     def validate(self) -> None:
         _require(self.gamma >= 0.0, f"Coupling gamma must be non-negative, got {self.gamma}.")
+        _require(isinstance(self.mpm_refinement, int) and self.mpm_refinement >= 1,
+                 f"mpm_refinement must be a positive integer, got {self.mpm_refinement}.")
         _require(self.mpm_substeps_per_lbm_step > 0, "mpm_substeps_per_lbm_step must be positive.")
         _require(self.kernel in _COUPLING_KERNELS, f"Unsupported coupling kernel: {self.kernel}.")
         _require(self.force_limit is None or self.force_limit > 0.0,
@@ -222,9 +230,15 @@ class SimulationConfig:
         self.output.validate()
         lbm_dims = (self.lbm.nx, self.lbm.ny, self.lbm.nz)
         mpm_dims = (self.mpm.nx, self.mpm.ny, self.mpm.nz)
-        _require(lbm_dims == mpm_dims,
-                 "For MVP, LBM and MPM grids must have the same resolution. "
-                 f"Got LBM={lbm_dims}, MPM={mpm_dims}.")
+        r = self.coupling.mpm_refinement
+        if r == 1:
+            _require(lbm_dims == mpm_dims,
+                     "For MVP, LBM and MPM grids must have the same resolution. "
+                     f"Got LBM={lbm_dims}, MPM={mpm_dims}.")
+        else:
+            _require(mpm_dims == tuple(r * n for n in lbm_dims),
+                     f"With mpm_refinement={r} the MPM grid must be {r} x the LBM grid. "
+                     f"Got LBM={lbm_dims}, MPM={mpm_dims}.")
 
     @property
     def mpm_dt(self) -> float:

@@ -46,8 +46,11 @@ class LBMMpmCoupler:
         self.nx = int(lbm.nx)
         self.ny = int(lbm.ny)
         self.nz = int(lbm.nz)
-        self.inv_dx = 1.0 / float(mpm.dx)
-        self.cell_volume = float(mpm.dx) ** 3
+        # the lattice cell the coupling works on: the MPM cell (the
+        # reference), or mpm_refinement MPM cells per axis
+        self.refine = int(config.mpm_refinement)
+        self.inv_dx = 1.0 / (self.refine * float(mpm.dx))
+        self.cell_volume = (self.refine * float(mpm.dx)) ** 3
 
         if lbm.real_dtype != mpm.real_dtype:
             raise ValueError("LBM and MPM solvers must use the same real type for coupling.")
@@ -89,6 +92,8 @@ class LBMMpmCoupler:
         and adds its correction to every coupled particle's penalty force
         (and so to the fluid's reaction); particles outside the training
         envelope keep the plain force. Not part of the reference."""
+        if closure is not None and self.refine > 1:
+            raise NotImplementedError("the learned closure needs co-located grids (mpm_refinement 1).")
         self.closure = closure
         if closure is None:
             self.closure_force = None
@@ -131,12 +136,18 @@ class LBMMpmCoupler:
         # fsi/coupling.py:42-56
         if self.config.kernel != "quadratic":
             raise ValueError(f"Unsupported coupling kernel: {self.config.kernel}.")
-        if (self.lbm.nx, self.lbm.ny, self.lbm.nz) != (self.mpm.nx, self.mpm.ny, self.mpm.nz):
+        r = int(self.config.mpm_refinement)
+        lbm_dims = (self.lbm.nx, self.lbm.ny, self.lbm.nz)
+        mpm_dims = (self.mpm.nx, self.mpm.ny, self.mpm.nz)
+        if r == 1 and lbm_dims != mpm_dims:
             raise ValueError(
                 "LBM and MPM grids must match for Step 5 coupling. "
-                f"Got LBM={(self.lbm.nx, self.lbm.ny, self.lbm.nz)}, "
-                f"MPM={(self.mpm.nx, self.mpm.ny, self.mpm.nz)}."
+                f"Got LBM={lbm_dims}, "
+                f"MPM={mpm_dims}."
             )
+        if r > 1 and mpm_dims != tuple(r * n for n in lbm_dims):
+            raise ValueError(f"With mpm_refinement={r} the MPM grid must be {r} x the LBM grid. "
+                             f"Got LBM={lbm_dims}, MPM={mpm_dims}.")
         if self.mpm.dx <= 0.0:
             raise ValueError("MPM dx must be positive.")
 
@@ -174,6 +185,7 @@ class LBMMpmCoupler:
         p.gamma_eff = self._kernel_gamma()
         p.dt_ratio = float(self._np_real(dt_ratio))
         p.cell_volume = float(self._np_real(self.cell_volume))
+        p.refine = self.refine
         p.min_valid_weight = float(c.min_valid_weight)
         p.use_relative_velocity_limit = 0 if c.relative_velocity_limit is None else 1
         p.relative_velocity_limit = 0.0 if c.relative_velocity_limit is None else float(c.relative_velocity_limit)

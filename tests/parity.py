@@ -127,6 +127,36 @@ def coupled_scene(storage=None, n=(24, 16, 16), substeps=8, gamma=0.5, e_mod=400
     return sim_cfg, solid, pos, vel
 
 
+def refined_scene(storage=None, refine=2, n=(16, 12, 12), substeps=16, ib=False, contact=False,
+                  seed=0, wall=False):
+    """The coupled scene with the MPM grid ``refine`` times finer than the
+    lattice (MPM dx = 1/refine lattice cells, 2 particles per MPM cell and
+    axis): a jittered cube of side 5 lattice cells in the channel, spinning;
+    ``wall`` puts it against the solid shell at y = 0 (partial support).
+    ``refine=1`` gives the same particles on a co-located MPM grid."""
+    nx, ny, nz = n
+    lbm_cfg, solid = channel_config(nx, ny, nz, storage=storage, initial_velocity=(0.02, 0.0, 0.0))
+    solid[:, 1:-1, 1:-1] = 0
+    dx = 1.0 / refine
+    mpm_cfg = MPMConfig(nx=refine * nx, ny=refine * ny, nz=refine * nz, max_particles=8192, dx=dx,
+                        dt=0.1, density=1.2, youngs_modulus=400.0, poisson_ratio=0.3,
+                        gravity=(0.0, 0.0, 0.0), boundary_width=2)
+    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=substeps, mpm_refinement=refine,
+                         immersed_boundary_enabled=ib, immersed_boundary_drag=0.05 if ib else 0.0,
+                         immersed_boundary_fraction_threshold=0.05,
+                         contact_enabled=contact, contact_velocity_damping=0.1 if contact else 0.0,
+                         contact_fraction_threshold=0.9)
+    sim_cfg = SimulationConfig(num_steps=10, lbm_dt=0.1, lbm=lbm_cfg, mpm=mpm_cfg, coupling=cpl)
+    c = np.array([nx * 0.35, 3.8 if wall else ny / 2, nz / 2])
+    pos = jittered_cube(c - 2.5, c + 2.5, 0.25, seed)
+    omega = np.array([0.0, 0.01, 0.02])
+    vel = np.cross(omega, pos - c).astype(np.float32)
+    return sim_cfg, solid, pos, vel
+
+
+REFINED_MASS, REFINED_VOLUME = 1.2 * 0.25 ** 3, 0.25 ** 3
+
+
 def sim_fields(sim) -> dict:
     out = lbm_fields(sim.lbm)
     out.update(

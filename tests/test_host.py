@@ -70,6 +70,12 @@ def test_defaults_match_reference():
         SimulationConfig(num_steps=0),
         SimulationConfig(lbm_dt=0.0),
         SimulationConfig(mpm=MPMConfig(nx=65)),
+        CouplingConfig(mpm_refinement=0),
+        CouplingConfig(mpm_refinement=1.5),
+        # refined MPM grid: every axis r times the lattice's
+        SimulationConfig(coupling=CouplingConfig(mpm_refinement=2)),
+        SimulationConfig(coupling=CouplingConfig(mpm_refinement=2),
+                         mpm=MPMConfig(nx=128, ny=128, nz=64)),
     ],
 )
 def test_invalid_config_raises_value_error(cfg):

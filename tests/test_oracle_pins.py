@@ -414,3 +414,39 @@ def test_positions_clamp_to_the_domain():
     assert np.array_equal(s.x[:n], np.minimum(np.maximum(free, lo), hi))
     assert np.any(free < lo) and np.any(free > hi)
     assert np.all(s.x[:n] >= lo) and np.all(s.x[:n] <= hi)
+
+
+def _oracle_coupling(refine, contact):
+    from parity import REFINED_MASS, REFINED_VOLUME, refined_scene
+
+    cfg, solid, pos, vel = refined_scene(refine=refine, contact=contact, ib=True)
+    s = po.OracleSimulation(cfg, "f64")
+    s.initialize_lbm(solid)
+    s.initialize_mpm_from_numpy(pos, vel, particle_mass=REFINED_MASS, particle_volume=REFINED_VOLUME)
+    s.coupler.compute_coupling_forces(0.25)
+    c = s.coupler
+    return {
+        "cforce": c.coupling_force_numpy(), "phi": c.solid_volume_fraction_numpy(),
+        "ibf": c.immersed_boundary_force_numpy(), "pforce": s.mpm.particle_forces_numpy(),
+        "wvalid": c.particle_valid_weights_numpy(), "cmask": c.particle_coupling_mask_numpy(),
+        "contact": c.particle_contact_mask_numpy(), "counters": np.array(c.counters).copy(),
+    }
+
+
+def test_refined_grid_couples_on_the_lattice():
+    """The coupling (fsi/coupling.py:463-496) reads the lattice and the
+    particles only: with the MPM grid twice finer than the lattice, the
+    same particles give the same coupling force, solid fraction, particle
+    forces, masks and counters as on a co-located MPM grid, and an MPM-grid
+    stencil (node spacing half a lattice cell) would not."""
+    a = _oracle_coupling(2, True)
+    b = _oracle_coupling(1, True)
+    assert int(a["counters"][3]) > 0 and float(np.abs(a["cforce"]).max()) > 0
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    # the fluid reaction balances the particle forces, weighted by the
+    # substep's share of the lattice step (dt_ratio 0.25; all cells are fluid
+    # where the cube sits)
+    tot_p = a["pforce"].sum(axis=0)
+    tot_f = a["cforce"].reshape(-1, 3).sum(axis=0)
+    assert np.abs(0.25 * tot_p + tot_f).max() < 1e-12 * np.abs(tot_p).max()
