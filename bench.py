@@ -174,7 +174,7 @@ def _cpu_task(task, budget_s):
         cfg, solid, pos = coupled_scene_32()
         sim = po.OracleSimulation(cfg, "f32")
         sim.initialize_lbm(solid)
-        sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+        sim.initialize_mpm_from_numpy(pos, particle_volume=particle_volume(cfg))
         sim.step()
         done, t0 = 0, time.perf_counter()
         while done < 20 and (done == 0 or time.perf_counter() - t0 < budget_s):
@@ -189,12 +189,12 @@ def _cpu_task(task, budget_s):
         # the reference's one global material; the cost does not depend on E)
         import dataclasses
 
-        cfg, solid, pos, _ = coupled_scene_large()
+        cfg, solid, pos, _ = coupled_scene_large(refine=LARGE_REFINE)
         sub = cfg.coupling.mpm_substeps_per_lbm_step
         cfg2 = dataclasses.replace(cfg, coupling=dataclasses.replace(cfg.coupling, mpm_substeps_per_lbm_step=2))
         sim = po.OracleSimulation(cfg2, "f32")
         sim.initialize_lbm(solid)
-        sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+        sim.initialize_mpm_from_numpy(pos, particle_volume=particle_volume(cfg))
         t0 = time.perf_counter()
         sim.lbm.step()
         t_lat = time.perf_counter() - t0
@@ -246,7 +246,8 @@ def cpu_baseline(budget_s=10.0):
                            "sample": f"{lat1.get('steps')} timed steps of the same lattice"},
         "coupled_32": {"value": round(c32.get("value", float("nan")), 4), "unit": "coupled steps/s",
                        "cores": int(c32.get("threads", cores)),
-                       "sample": f"{c32.get('steps')} coupled steps of the 32^3 scene (4096 particles, 112 substeps)"},
+                       "sample": f"{c32.get('steps')} coupled steps of the 32^3 scene (4096 particles, MPM grid "
+                                 f"64^3 at dx_mpm = dx_lbm / 2, 224 substeps)"},
         "coupled_32_1_core": {"value": round(c32_1.get("value", float("nan")), 4), "unit": "coupled steps/s",
                               "cores": 1, "sample": f"{c32_1.get('steps')} coupled steps"},
         "coupled_large": {"value": round(large.get("value", float("nan")), 5), "unit": "coupled steps/s",
@@ -257,47 +258,71 @@ def cpu_baseline(budget_s=10.0):
     return main, more
 
 
-def coupled_scene_32(storage="fp32"):
+def coupled_scene_32(storage="fp32", refine=2):
     """32^3 coupled penalty scene: inflow u_x=0.02 (velocity face x-), pressure
-    face x+, no-slip walls as a solid shell in y/z, tau=0.6; elastic cube
-    [8,16]x[12,20]^2 of 16^3 particles (spacing 0.5, jitter U(-0.125,0.125),
-    seed 0), E=1e3, nu=0.3, density 1.2, gamma=0.5, 112 substeps."""
+    face x+, no-slip walls as a solid shell in y/z, tau=0.6; an elastic cube
+    of 8^3 MPM cells with 8 particles per cell (16^3 = 4096 particles,
+    spacing dx_mpm / 2, jitter U(-1/4, 1/4) of the spacing, seed 0) centred
+    at (12, 16, 16), E=1e3, nu=0.3, density 1.2, gamma=0.5. refine=2: the MPM
+    grid is 64^3 at dx_mpm = dx_lbm / 2 (the cube spans [10, 14] x [14, 18]^2,
+    224 substeps); refine=1: co-located, the cube spans [8, 16] x [12, 20]^2,
+    112 substeps (P-wave Courant number 0.3 either way)."""
+    import numpy as np
+
     from fsi.config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
     from fsi.scenes import channel_walls, jittered_box
 
     n = 32
+    dx = 1.0 / refine
+    sub = 112 * refine
     lbm = LBMConfig(nx=n, ny=n, nz=n, viscosity=0.1 / 3.0, rho0=1.0,
                     x_left=BoundaryConfig("velocity", velocity=(0.02, 0.0, 0.0)),
                     x_right=BoundaryConfig("pressure", rho=1.0), storage=storage)
-    mpm = MPMConfig(nx=n, ny=n, nz=n, max_particles=4096, dx=1.0, dt=1.0 / 112, density=1.2,
-                    youngs_modulus=1.0e3, poisson_ratio=0.3, gravity=(0.0, 0.0, 0.0),
+    mpm = MPMConfig(nx=refine * n, ny=refine * n, nz=refine * n, max_particles=4096, dx=dx, dt=1.0 / sub,
+                    density=1.2, youngs_modulus=1.0e3, poisson_ratio=0.3, gravity=(0.0, 0.0, 0.0),
                     boundary_width=2)
-    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=112)
+    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=sub, mpm_refinement=refine)
     cfg = SimulationConfig(num_steps=100, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
     solid = channel_walls((n, n, n))
-    pos = jittered_box((8.0, 12.0, 12.0), (16.0, 20.0, 20.0), 0.5, seed=0, jitter=0.5)
+    h = 4.0 * dx  # half the cube's side: 8 MPM cells
+    centre = np.array([12.0, 16.0, 16.0])
+    pos = jittered_box(tuple(centre - h), tuple(centre + h), 0.5 * dx, seed=0, jitter=0.5)
     return cfg, solid, pos
 
 
+def particle_volume(cfg):
+    """Volume of one of the 8 particles per MPM cell of these scenes."""
+    return (0.5 * cfg.mpm.dx) ** 3
+
+
 LARGE_SUBSTEPS = 256
+LARGE_REFINE = 2  # the suspension as stated: dx_mpm = dx_lbm / 2
 
 
-def coupled_scene_large(storage="fp32"):
+def large_substeps(refine=1):
+    """Substeps per lattice step of the suspension: the stiffest sphere's
+    P-wave speed sqrt((lam + 2 mu)/rho) is 74.9 lattice cells per time unit,
+    and mpm_dt <= 0.3 dx_mpm / c_p needs >= 250 substeps at dx_mpm = dx_lbm
+    (256 here), twice that at dx_mpm = dx_lbm / 2."""
+    return LARGE_SUBSTEPS * refine
+
+
+def coupled_scene_large(storage="fp32", refine=1):
     """256x128x128 coupled suspension: inflow u_x=0.04 (velocity face x-,
     fluid starting at that velocity), pressure face x+, no-slip walls as a
     solid shell in y/z, tau=0.56; 16 non-overlapping elastic spheres r U[6,12]
-    (sphere_pack seed 0), 8 particles per cell (spacing 0.5, jitter
-    U(-0.125,0.125)), E log-uniform in [5e2, 5e3] per sphere (seed 0, a
-    per-particle material plane), nu=0.3, density 1.2, gamma=0.5.
-    Substeps: the stiffest sphere's P-wave speed sqrt((lam + 2 mu)/rho) is
-    74.9 cells per time unit, and mpm_dt <= 0.3 dx / c_p needs >= 250
-    substeps per lattice step (256 here).
+    lattice cells (sphere_pack seed 0), 8 particles per MPM cell (spacing
+    dx_mpm / 2, jitter U(-1/4, 1/4) of the spacing), E log-uniform in
+    [5e2, 5e3] per sphere (seed 0, a per-particle material plane), nu=0.3,
+    density 1.2, gamma=0.5. refine=2: the MPM grid is 512x256x256 at
+    dx_mpm = dx_lbm / 2 (the configuration as stated), refine=1 co-located.
     Returns (config, solid mask, positions, per-particle E)."""
     from fsi.config import BoundaryConfig, CouplingConfig, LBMConfig, MPMConfig, SimulationConfig
     from fsi.scenes import body_moduli, channel_walls, sphere_pack
 
     shape = (256, 128, 128)
-    pos, info = sphere_pack(shape, 16, 6.0, 12.0, spacing=0.5, seed=0)
+    dx = 1.0 / refine
+    pos, info = sphere_pack(shape, 16, 6.0, 12.0, spacing=0.5 * dx, seed=0)
     E, _ = body_moduli(info["counts"], 5.0e2, 5.0e3, seed=0)
     # the fluid starts at the inflow velocity: started from rest, the
     # reference's velocity face on this channel becomes unstable after ~140
@@ -306,31 +331,38 @@ def coupled_scene_large(storage="fp32"):
                     initial_velocity=(0.04, 0.0, 0.0),
                     x_left=BoundaryConfig("velocity", velocity=(0.04, 0.0, 0.0)),
                     x_right=BoundaryConfig("pressure", rho=1.0), storage=storage)
-    mpm = MPMConfig(nx=shape[0], ny=shape[1], nz=shape[2], max_particles=len(pos), dx=1.0,
-                    dt=1.0 / LARGE_SUBSTEPS, density=1.2, youngs_modulus=1.0e3, poisson_ratio=0.3,
+    sub = large_substeps(refine)
+    mpm = MPMConfig(nx=refine * shape[0], ny=refine * shape[1], nz=refine * shape[2], max_particles=len(pos),
+                    dx=dx, dt=1.0 / sub, density=1.2, youngs_modulus=1.0e3, poisson_ratio=0.3,
                     gravity=(0.0, 0.0, 0.0), boundary_width=2)
-    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=LARGE_SUBSTEPS)
+    cpl = CouplingConfig(gamma=0.5, mpm_substeps_per_lbm_step=sub, mpm_refinement=refine)
     cfg = SimulationConfig(num_steps=2000, lbm_dt=1.0, lbm=lbm, mpm=mpm, coupling=cpl)
     return cfg, channel_walls(shape), pos, E
 
 
-def large_sim(storage):
+def large_workload(refine):
+    n = "512x256x256 MPM grid at dx_mpm = dx_lbm / 2" if refine == 2 else "co-located 256x128x128 MPM grid"
+    return (f"256x128x128 coupled suspension, 16 elastic spheres (E log-uniform [5e2, 5e3] per sphere), "
+            f"{n}, 8 particles per MPM cell, {large_substeps(refine)} substeps")
+
+
+def large_sim(storage, refine=1):
     import fsi
 
-    cfg, solid, pos, E = coupled_scene_large(storage)
+    cfg, solid, pos, E = coupled_scene_large(storage, refine)
     sim = fsi.FSISimulation(cfg)
     sim.initialize_lbm(solid)
-    sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+    sim.initialize_mpm_from_numpy(pos, particle_volume=(0.5 / refine) ** 3)
     sim.mpm.set_particle_material(E)
     return sim, cfg, pos
 
 
-def time_coupled_large(steps, storage="fp32", min_warmup_s=0.5):
+def time_coupled_large(steps, storage="fp32", min_warmup_s=0.5, refine=1):
     """Coupled steps/s of the suspension: warm-up of at least two steps and
     min_warmup_s seconds, then `steps` timed steps."""
     import torch
 
-    sim, cfg, pos = large_sim(storage)
+    sim, cfg, pos = large_sim(storage, refine)
     warm_up(sim.advance, 2, min_warmup_s)
     wall, _ = timed_loop(sim.advance, steps)
     n_sub = cfg.coupling.mpm_substeps_per_lbm_step
@@ -339,7 +371,7 @@ def time_coupled_large(steps, storage="fp32", min_warmup_s=0.5):
             "particle_substeps_per_s": round(len(pos) * n_sub * steps / wall, 1)}
 
 
-def large_errors_vs_fp64(steps, storages=("fp32", "i16")):
+def large_errors_vs_fp64(steps, storages=("fp32", "i16"), refine=1):
     """The suspension after `steps` coupled steps from the same start in the
     FP32 build and the mixed-precision build (I16 lattice populations, FP32
     MPM and coupling), against the FP64 build: max-norm relative error of the
@@ -353,14 +385,14 @@ def large_errors_vs_fp64(steps, storages=("fp32", "i16")):
 
     from fsi.scenes import sphere_pack
 
-    _, info = sphere_pack((256, 128, 128), 16, 6.0, 12.0, spacing=0.5, seed=0)
+    _, info = sphere_pack((256, 128, 128), 16, 6.0, 12.0, spacing=0.5 / refine, seed=0)
     bounds = np.concatenate([[0], np.cumsum(info["counts"])])
 
     def centroids(x):
         return np.stack([x[bounds[i]:bounds[i + 1]].mean(0) for i in range(len(bounds) - 1)])
 
     def run(storage):
-        sim, _, pos = large_sim(storage)
+        sim, _, pos = large_sim(storage, refine)
         sim.coupler.record_step_force()
         t0 = time.perf_counter()
         for i in range(steps):
@@ -560,15 +592,15 @@ def closure_large_scene(closure_path, steps=2000, speed_steps=2, plain_storages=
                       "speedup": round(tf / tc, 2)}}
 
 
-def time_coupled(steps, warmup):
+def time_coupled(steps, warmup, refine=2):
     import torch
 
     import fsi
 
-    cfg, solid, pos = coupled_scene_32()
+    cfg, solid, pos = coupled_scene_32(refine=refine)
     sim = fsi.FSISimulation(cfg)
     sim.initialize_lbm(solid)
-    sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+    sim.initialize_mpm_from_numpy(pos, particle_volume=particle_volume(cfg))
     for _ in range(warmup):
         sim.advance()
     torch.cuda.synchronize()
@@ -578,7 +610,7 @@ def time_coupled(steps, warmup):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return {"steps_per_s": round(steps / dt, 3), "ms_per_step": round(1e3 * dt / steps, 4),
-            "particles": int(len(pos)), "substeps": 112, "timed_steps": steps}
+            "particles": int(len(pos)), "substeps": cfg.coupling.mpm_substeps_per_lbm_step, "timed_steps": steps}
 
 
 def calibrate_copy(n):
@@ -622,6 +654,10 @@ def main():
                     help="coupled steps of the (untimed) suspension runs compared with its FP64 build")
     ap.add_argument("--large-only", action="store_true",
                     help="only the suspension: timing and the FP64 comparison (its own JSON line)")
+    ap.add_argument("--large-refine", type=int, default=LARGE_REFINE, choices=(1, 2),
+                    help="MPM cells per lattice cell and axis of the timed suspension (2: dx_mpm = dx_lbm / 2)")
+    ap.add_argument("--large-error-refine", type=int, default=1, choices=(1, 2),
+                    help="the same for the suspension's FP64 comparison")
     ap.add_argument("--closure-config", choices=("train", "large"), default=None,
                     help="only the learned closure at its configuration's sizes (its own JSON line): 'train' "
                          "generates the coarse/refined pair data and trains; 'large' runs the 64-sphere scene")
@@ -652,11 +688,14 @@ def main():
         print(json.dumps(closure_large_scene(args.closure_path, steps=args.closure_steps or 2000)), flush=True)
         return
     if args.large_only:
-        out = {"workload": f"256x128x128 coupled suspension, 16 elastic spheres (E log-uniform [5e2, 5e3] per "
-                           f"sphere), {LARGE_SUBSTEPS} substeps",
-               "fp32": time_coupled_large(args.large_steps), "mixed_i16": time_coupled_large(args.large_steps, "i16")}
+        r = args.large_refine
+        out = {"workload": large_workload(r)}
+        if args.large_steps > 0:
+            out["fp32"] = time_coupled_large(args.large_steps, refine=r)
+            out["mixed_i16"] = time_coupled_large(args.large_steps, "i16", refine=r)
         if args.large_error_steps > 0:
-            out["error_vs_fp64"] = large_errors_vs_fp64(args.large_error_steps)
+            out["error_vs_fp64"] = {"workload": large_workload(args.large_error_refine),
+                                    **large_errors_vs_fp64(args.large_error_steps, refine=args.large_error_refine)}
         print(json.dumps(out), flush=True)
         return
 
@@ -757,21 +796,29 @@ def main():
     if line_errors is not None:
         line["error_vs_fp64"] = line_errors
     if not args.no_coupled:
-        line["coupled"] = {"workload": "32^3 coupled penalty scene, FP32, 4096 particles, 112 substeps",
+        line["coupled"] = {"workload": "32^3 coupled penalty scene, FP32, 4096 particles, MPM grid 64^3 at "
+                                       "dx_mpm = dx_lbm / 2, 224 substeps",
                            **time_coupled(args.coupled_steps, 2)}
+        line["coupled_colocated"] = {"workload": "the same cube on a co-located 32^3 MPM grid (twice the "
+                                                 "particle spacing), 112 substeps",
+                                     **time_coupled(args.coupled_steps, 2, refine=1)}
     if args.large_steps > 0:
-        line["coupled_large"] = {
-            "workload": f"256x128x128 coupled suspension, 16 elastic spheres (E log-uniform [5e2, 5e3] per sphere), "
-                        f"FP32, {LARGE_SUBSTEPS} substeps",
-            **time_coupled_large(args.large_steps)}
+        r = args.large_refine
+        line["coupled_large"] = {"workload": large_workload(r) + ", FP32",
+                                 **time_coupled_large(args.large_steps, refine=r)}
         # mixed precision: I16 lattice populations, FP32 MPM and coupling
         line["coupled_large_mixed"] = {
-            "workload": f"the same suspension, I16 lattice storage, FP32 MPM and coupling, {LARGE_SUBSTEPS} substeps",
-            **time_coupled_large(args.large_steps, storage="i16")}
+            "workload": "the same suspension, I16 lattice storage, FP32 MPM and coupling",
+            **time_coupled_large(args.large_steps, storage="i16", refine=r)}
         line["coupled_large_mixed"]["speedup_vs_fp32"] = round(
             line["coupled_large_mixed"]["steps_per_s"] / line["coupled_large"]["steps_per_s"], 3)
+        if r != 1:
+            line["coupled_large_colocated"] = {"workload": large_workload(1) + ", FP32",
+                                               **time_coupled_large(args.large_steps, refine=1)}
         if not args.no_error_check and args.large_error_steps > 0:
-            line["coupled_large_error_vs_fp64"] = large_errors_vs_fp64(args.large_error_steps)
+            line["coupled_large_error_vs_fp64"] = {
+                "workload": large_workload(args.large_error_refine),
+                **large_errors_vs_fp64(args.large_error_steps, refine=args.large_error_refine)}
     if not args.no_closure:
         line["learned_closure"] = time_closure()
     if not args.no_cpu_baseline:

@@ -280,27 +280,36 @@ def test_coupled_mixed_precision_error(coupled_oracle):
     assert np.array_equal(got["cmask"], ref["cmask"])
 
 
-@pytest.fixture(scope="module")
-def long_run_oracle():
+_long_runs = {}
+
+
+@pytest.fixture(scope="module", params=[2, 1], ids=["dx_half", "colocated"])
+def long_run_oracle(request):
     import bench
 
-    cfg, solid, pos = bench.coupled_scene_32()
-    out = {}
-    for p in ("f32", "f64"):
-        sim = po.OracleSimulation(cfg, p)
-        sim.initialize_lbm(solid)
-        sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
-        for _ in range(100):
-            sim.step()
-        out[p] = sim_fields(sim)
-    return out
+    refine = request.param
+    if refine not in _long_runs:
+        _long_runs.clear()
+        cfg, solid, pos = bench.coupled_scene_32(refine=refine)
+        out = {"refine": refine}
+        for p in ("f32", "f64"):
+            sim = po.OracleSimulation(cfg, p)
+            sim.initialize_lbm(solid)
+            sim.initialize_mpm_from_numpy(pos, particle_volume=bench.particle_volume(cfg))
+            for _ in range(100):
+                sim.step()
+            out[p] = sim_fields(sim)
+        _long_runs[refine] = out
+    return _long_runs[refine]
 
 
 @pytest.mark.timeout(1200)
 @pytest.mark.parametrize("storage", ["fp32", "fp64"])
 def test_coupled_32_scene_100_steps(storage, long_run_oracle):
     """The reference's CPU-runnable coupled case (32^3 lattice, 4096
-    particles, 112 substeps) over 100 coupled steps (11,200 MPM substeps).
+    particles) over 100 coupled steps: as stated, on a 64^3 MPM grid at
+    dx_mpm = dx_lbm / 2 with 224 substeps (22,400 MPM substeps), and with the
+    same cube on a co-located grid (112 substeps).
 
     FP64 build against the FP64 oracle within 1e-7 relative per field: the
     only difference is the order of the float atomics, amplified by the run
@@ -313,10 +322,11 @@ def test_coupled_32_scene_100_steps(storage, long_run_oracle):
     import bench
     import fsi
 
-    cfg, solid, pos = bench.coupled_scene_32(storage)
+    refine = long_run_oracle["refine"]
+    cfg, solid, pos = bench.coupled_scene_32(storage, refine=refine)
     sim = fsi.FSISimulation(cfg)
     sim.initialize_lbm(solid)
-    sim.initialize_mpm_from_numpy(pos, particle_volume=0.125)
+    sim.initialize_mpm_from_numpy(pos, particle_volume=bench.particle_volume(cfg))
     for _ in range(100):
         sim.step()
     got = sim_fields(sim)
@@ -326,7 +336,8 @@ def test_coupled_32_scene_100_steps(storage, long_run_oracle):
         e = rel_inf(got[key], ref64[key])
         e32 = rel_inf(ref32[key], ref64[key])
         e_vs32 = rel_inf(got[key], ref32[key])
-        print(f"100 steps {storage} {key}: vs f64 {e:.3e}, f32 oracle vs f64 {e32:.3e}, vs f32 oracle {e_vs32:.3e}")
+        print(f"100 steps r={refine} {storage} {key}: vs f64 {e:.3e}, f32 oracle vs f64 {e32:.3e}, "
+              f"vs f32 oracle {e_vs32:.3e}")
         bound = 1e-7 if storage == "fp64" else 4.0 * e32 + 1e-6
         if not e <= bound:
             bad.append((key, e, bound))

@@ -1,6 +1,8 @@
 """Properties of the full-size coupled suspension (256x128x128 lattice, 16
-elastic spheres of log-uniform E, 448,861 particles, 256 substeps per
-lattice step; bench.coupled_scene_large), on the large-set kernel variants.
+elastic spheres of log-uniform E; bench.coupled_scene_large), on the
+large-set kernel variants: on a co-located MPM grid (448,861 particles, 256
+substeps per lattice step) and on the 512x256x256 MPM grid at
+dx_mpm = dx_lbm / 2 (3,591,333 particles, 512 substeps).
 
 No oracle runs at this size; the checks are the ones the physics fixes:
 
@@ -24,10 +26,13 @@ pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("gpu_available")]
 STEPS = 8
 
 
-def _run(storage):
+PARTICLES = {1: 448861, 2: 3591333}
+
+
+def _run(storage, refine):
     import bench
 
-    sim, cfg, pos = bench.large_sim(storage)
+    sim, cfg, pos = bench.large_sim(storage, refine)
     sim.coupler.record_step_force()
     m0 = sim.diagnostics()["lbm_total_mass"]
     x0 = sim.mpm.positions_numpy().astype(np.float64)
@@ -50,9 +55,17 @@ def _run(storage):
     return out
 
 
-@pytest.fixture(scope="module")
-def runs():
-    return {s: _run(s) for s in ("fp64", "fp32", "i16")}
+_cache = {}
+
+
+@pytest.fixture(scope="module", params=[1, 2], ids=["colocated", "dx_half"])
+def runs(request):
+    refine = request.param
+    if refine not in _cache:
+        _cache.clear()  # one scene's runs held at a time
+        _cache[refine] = {s: _run(s, refine) for s in ("fp64", "fp32", "i16")}
+        _cache[refine]["refine"] = refine
+    return _cache[refine]
 
 
 @pytest.mark.parametrize("storage", ["fp64", "fp32", "i16"])
@@ -69,7 +82,7 @@ def test_coupling_momentum_balance(runs, storage):
 @pytest.mark.parametrize("storage", ["fp64", "fp32", "i16"])
 def test_flow_and_bodies_stay_bounded(runs, storage):
     r = runs[storage]
-    assert r["particles"] == 448861 and r["unsupported"] == 0
+    assert r["particles"] == PARTICLES[runs["refine"]] and r["unsupported"] == 0
     for k in ("u", "v", "x"):
         assert np.all(np.isfinite(r[k])), k
     umax = np.linalg.norm(r["u"], axis=-1).max()
