@@ -5,7 +5,8 @@ counter-row scratch in the library's static device array. Two coupled scenes
 in a row (the second with fewer particles, so the static array holds the
 first call's larger contents when it starts): every coupled step's substeps
 go through one entry point or the other, and the FP64 fields must agree to
-rounding of the float atomics' order (1e-10 relative), counters exactly.
+rounding of the float atomics' order (1e-10 relative), counters exactly. A
+third scene runs the MPM grid at half the lattice spacing.
 """
 
 import ctypes
@@ -13,7 +14,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from parity import close, coupled_scene, jittered_cube, sim_fields
+from parity import REFINED_MASS, REFINED_VOLUME, close, coupled_scene, jittered_cube, refined_scene, sim_fields
 
 pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("gpu_available")]
 
@@ -29,14 +30,14 @@ def _two_cube_scene():
     return cfg, solid, pos, vel
 
 
-def _run(scene, entry, steps):
+def _run(scene, entry, steps, mass=1.2 * 0.125, volume=0.125):
     import fsi
     from fsi import _native as nat
 
     cfg, solid, pos, vel = scene
     sim = fsi.FSISimulation(cfg)
     sim.initialize_lbm(solid)
-    sim.initialize_mpm_from_numpy(pos, vel, particle_mass=1.2 * 0.125, particle_volume=0.125)
+    sim.initialize_mpm_from_numpy(pos, vel, particle_mass=mass, particle_volume=volume)
     real_call = nat.call
     seen = []
 
@@ -62,11 +63,16 @@ FIELDS = ("dev", "rho", "u", "x", "v", "F", "pforce", "cforce", "phi", "lbm_forc
 
 
 def test_noalloc_substeps_match_the_allocating_entry_point():
-    scenes = [("two cubes", _two_cube_scene()), ("one cube", coupled_scene(storage="fp64", ib=True, contact=True))]
+    plain = {}
+    scenes = [("two cubes", _two_cube_scene(), plain),
+              ("one cube", coupled_scene(storage="fp64", ib=True, contact=True), plain),
+              # the MPM grid at half the lattice spacing (the refined kernels)
+              ("refined", refined_scene(storage="fp64", ib=True, contact=True, wall=True),
+               {"mass": REFINED_MASS, "volume": REFINED_VOLUME})]
     assert len(scenes[1][1][2]) < len(scenes[0][1][2])
-    for label, scene in scenes:
-        ref = _run(scene, "fsi_mpm_substeps", 3)
-        got = _run(scene, "fsi_mpm_substeps_noalloc", 3)
+    for label, scene, kw in scenes:
+        ref = _run(scene, "fsi_mpm_substeps", 3, **kw)
+        got = _run(scene, "fsi_mpm_substeps_noalloc", 3, **kw)
         for key in FIELDS:
             ok, err, diff = close(got[key], ref[key], 1e-10, 1e-14)
             print(f"noalloc {label} {key}: rel {err:.3e} abs {diff:.3e}", flush=True)
