@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FSI_ABI_VERSION 4
+#define FSI_ABI_VERSION 5
 
 enum {
   FSI_OK = 0,
@@ -88,7 +88,17 @@ typedef struct fsi_lbm_buffers {
                               fraction of the 16-bit range, and a fourth slot for a measuring
                               pass (floats [6240, 8320)); each step adapts the scale from the
                               previous write's maximum. Unused otherwise (may be null) */
+  uint32_t* solid_bits;    /* optional (may be null), used when nz % 64 == 0 with FP32 or FP64
+                              storage: FSI_SOLID_BITS_WORDS(nx, ny, nz) words written by
+                              fsi_lbm_init, the solid flags of each z-row as bits (bit b of
+                              row (i, j): cell k = b - 1, the periodic neighbours k = nz - 1 at
+                              b = 0 and k = 0 at b = nz + 1). The lattice step then reads a
+                              wavefront's 9 neighbour rows with scalar loads instead of the
+                              4-byte nbmask of every cell */
 } fsi_lbm_buffers;
+
+/* words of fsi_lbm_buffers.solid_bits: nx * ny rows of nz / 32 + 2 words */
+#define FSI_SOLID_BITS_WORDS(nx, ny, nz) ((int64_t)(nx) * (ny) * ((nz) / 32 + 2))
 
 /* Force seen by one step. When `field` is null, every fluid cell uses
  * `uniform` and no force memory is read. */

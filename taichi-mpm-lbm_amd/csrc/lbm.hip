@@ -393,7 +393,8 @@ template <> struct PullBlock<float> { static constexpr int value = 512; };
 template <typename S, int PHASE, bool FORCE_FIELD, bool FACES, bool ASM = false>
 __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
     LbmK<typename PopStore<S>::real> c, const S* __restrict__ pin, S* __restrict__ pout,
-    const uint32_t* __restrict__ nbmask, const typename PopStore<S>::real* __restrict__ force,
+    const uint32_t* __restrict__ nbmask, const uint32_t* __restrict__ sbits,
+    const typename PopStore<S>::real* __restrict__ force,
     typename PopStore<S>::real* __restrict__ rho_io, typename PopStore<S>::real* __restrict__ vel_io,
     const typename PopStore<S>::real* __restrict__ fv_prev,
     typename PopStore<S>::real* __restrict__ fv_next, AsmArgs<typename PopStore<S>::real> asm_args) {
@@ -422,14 +423,45 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   const bool write = live && (!ADAPT || c.pass != 1);
   const uint32_t n = live ? n_lane : N - 1;
   constexpr uint32_t B = sizeof(S);
-  const uint32_t mask = nbmask[n];
-  const bool solid = (mask & 1u) != 0;
-
   const uint32_t nz = (uint32_t)c.nz, ny = (uint32_t)c.ny, nx = (uint32_t)c.nx;
   const uint32_t k = n % nz;
   const uint32_t ij = n / nz;
   const uint32_t j = ij % ny;
   const uint32_t i = ij / ny;
+  // neighbour-solid bits (bit q: the cell c + e_q is solid): from the row
+  // bitmap when given (FP32/FP64 with nz % 64 == 0, so that a wavefront lies
+  // in one z-row and starts at a multiple of 64: its 9 neighbour rows' words
+  // are uniform scalar loads, and a lane's 3 bits k-1..k+1 of each row come
+  // from one funnel shift), else the per-cell mask
+  uint32_t mask;
+  bool from_bits = false;
+  if constexpr (sizeof(S) >= 4) from_bits = sbits != nullptr;
+  if (from_bits) {
+    const uint32_t words = nz / 32u + 2u;
+    const uint32_t ijw = __builtin_amdgcn_readfirstlane(ij);
+    const uint32_t kw = __builtin_amdgcn_readfirstlane(k);
+    const uint32_t iw = ijw / ny, jw = ijw % ny;
+    const uint32_t ii[3] = {iw == 0 ? nx - 1 : iw - 1, iw, iw == nx - 1 ? 0u : iw + 1};
+    const uint32_t jj[3] = {jw == 0 ? ny - 1 : jw - 1, jw, jw == ny - 1 ? 0u : jw + 1};
+    const uint32_t kk = k - kw;
+    const uint32_t sh = kk & 31u;
+    const bool upper = kk >= 32u;
+    uint32_t win[3][3];  // bits k-1, k, k+1 of row (i + a - 1, j + b - 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const uint32_t* rp = sbits + (ii[a] * ny + jj[b]) * words + kw / 32u;
+        const uint32_t w0 = rp[0], w1 = rp[1], w2 = rp[2];
+        win[a][b] = __builtin_amdgcn_alignbit(upper ? w2 : w1, upper ? w1 : w0, sh);
+      }
+    mask = 0u;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) mask |= ((win[ex(q) + 1][ey(q) + 1] >> (1 + ez(q))) & 1u) << q;
+  } else {
+    mask = nbmask[n];
+  }
+  const bool solid = (mask & 1u) != 0;
   const uint32_t run = nz * B;                // byte size of one population run
   const uint32_t rstride = 19 * nz;           // elements per row block
   const uint32_t own = (ij * rstride + k) * B;  // population q at own + q * run
@@ -754,6 +786,26 @@ int check_params(const fsi_lbm_params* p) {
   return 0;
 }
 
+// Row bitmap of the solid flags (fsi_lbm_buffers.solid_bits): bit b of row
+// (i, j) is solid(i, j, b - 1), with the periodic neighbours k = nz - 1 at
+// b = 0 and k = 0 at b = nz + 1; one thread per word.
+__global__ __launch_bounds__(kBlock) void k_solid_bits(int64_t rows, int nz, const int32_t* __restrict__ solid,
+                                                      uint32_t* __restrict__ bits) {
+  const int words = nz / 32 + 2;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * words) return;
+  const int64_t row = t / words;
+  const int w = (int)(t % words);
+  uint32_t v = 0;
+  for (int tb = 0; tb < 32; ++tb) {
+    const int b = 32 * w + tb;
+    if (b > nz + 1) break;
+    const int k = b == 0 ? nz - 1 : (b == nz + 1 ? 0 : b - 1);
+    if (solid[row * nz + k] != 0) v |= 1u << tb;
+  }
+  bits[t] = v;
+}
+
 template <typename S>
 int lbm_init_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pa, void* pb,
                hipStream_t st) {
@@ -764,6 +816,12 @@ int lbm_init_t(const fsi_lbm_params* p, const fsi_lbm_buffers* b, void* pa, void
                      R(p->initial_velocity[0]), R(p->initial_velocity[1]),
                      R(p->initial_velocity[2]));
   FSI_TRY_LAUNCH("lbm_init");
+  if (b->solid_bits) {
+    const int64_t words = FSI_SOLID_BITS_WORDS(p->nx, p->ny, p->nz);
+    hipLaunchKernelGGL(k_solid_bits, dim3(blocks_for(words)), dim3(kBlock), 0, st, (int64_t)p->nx * p->ny, p->nz,
+                       b->solid, b->solid_bits);
+    FSI_TRY_LAUNCH("solid_bits");
+  }
   return 0;
 }
 
@@ -773,13 +831,14 @@ void launch_pull(const LbmK<typename PopStore<S>::real>& c, bool faces, const vo
                  hipStream_t st, AsmArgs<typename PopStore<S>::real> a = {}) {
   using R = typename PopStore<S>::real;
   const dim3 grid(blocks_for(c.N, PullBlock<S>::value)), block(PullBlock<S>::value);
+  const uint32_t* sbits = (b->solid_bits && sizeof(S) >= 4 && c.nz % 64 == 0) ? b->solid_bits : nullptr;
   if (faces)
     hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, true, ASM>), grid, block, 0, st, c, (const S*)pin,
-                       (S*)pout, b->nbmask, (const R*)force, (R*)b->rho, (R*)b->vel,
+                       (S*)pout, b->nbmask, sbits, (const R*)force, (R*)b->rho, (R*)b->vel,
                        (const R*)fv_prev, (R*)fv_next, a);
   else
     hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, false, ASM>), grid, block, 0, st, c, (const S*)pin,
-                       (S*)pout, b->nbmask, (const R*)force, (R*)b->rho, (R*)b->vel,
+                       (S*)pout, b->nbmask, sbits, (const R*)force, (R*)b->rho, (R*)b->vel,
                        (const R*)fv_prev, (R*)fv_next, a);
 }
 

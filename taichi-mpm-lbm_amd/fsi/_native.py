@@ -14,7 +14,7 @@ import torch
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libfsi_hip.so"
 # include/fsi_hip.h FSI_ABI_VERSION the structures below follow
-ABI_VERSION = 4
+ABI_VERSION = 5
 _lib = None
 
 POP_F32, POP_F16, POP_F64, POP_I16 = 0, 1, 2, 3
@@ -50,7 +50,12 @@ class LbmParams(ctypes.Structure):
 
 class LbmBuffers(ctypes.Structure):
     _fields_ = [("solid", _vp), ("nbmask", _vp), ("rho", _vp), ("vel", _vp), ("force", _vp),
-                ("face_vel", _vp), ("store_ctrl", _vp)]
+                ("face_vel", _vp), ("store_ctrl", _vp), ("solid_bits", _vp)]
+
+
+def solid_bits_words(nx: int, ny: int, nz: int) -> int:
+    """Words of LbmBuffers.solid_bits (FSI_SOLID_BITS_WORDS)."""
+    return nx * ny * (nz // 32 + 2)
 
 
 class LbmForce(ctypes.Structure):

@@ -107,11 +107,16 @@ class LBMSolver3D:
                            if self.storage == "i16" else None)
         self._epoch = 0  # population writes since initialize()
 
+        # FP32/FP64 lattices whose z-rows are whole wavefronts read the solid
+        # flags as a row bitmap (scalar loads per wavefront) instead of the
+        # 4-byte neighbour mask per cell; set to None before initialize() to
+        # keep the mask
+        self.solid_bits = (torch.zeros(nat.solid_bits_words(self.nx, self.ny, self.nz), dtype=torch.int32,
+                                       device=dev)
+                           if self.storage in ("fp32", "fp64") and self.nz % 64 == 0 else None)
+
         self._params = self._make_params()
-        self._buffers = nat.LbmBuffers(
-            nat.ptr(self.solid), nat.ptr(self.nbmask), nat.ptr(self.rho), nat.ptr(self.vel),
-            nat.ptr(self.force), nat.ptr(self.face_vel), nat.ptr(self.store_ctrl),
-        )
+        self._buffers = self._make_buffers()
         self._materialized = False
         self._face_slot = 0
         self._force_dirty = False
@@ -220,6 +225,7 @@ class LBMSolver3D:
         else:
             solid = self._normalize_solid_mask(solid_np)
         self.solid.copy_(torch.from_numpy(solid.reshape(-1)))
+        self._buffers = self._make_buffers()
         if self.store_ctrl is not None:
             nat.fill_zero(self.store_ctrl)
         self._epoch = 0
@@ -232,6 +238,12 @@ class LBMSolver3D:
         self._force_dirty = False
         self._initialized = True
         self.reset_force()
+
+    def _make_buffers(self) -> nat.LbmBuffers:
+        return nat.LbmBuffers(
+            nat.ptr(self.solid), nat.ptr(self.nbmask), nat.ptr(self.rho), nat.ptr(self.vel),
+            nat.ptr(self.force), nat.ptr(self.face_vel), nat.ptr(self.store_ctrl), nat.ptr(self.solid_bits),
+        )
 
     def _normalize_solid_mask(self, solid_np: np.ndarray) -> np.ndarray:
         arr = np.asarray(solid_np)

@@ -49,6 +49,11 @@ def _scenarios():
         x_right=dataclasses.replace(chan.x_right, boundary_type="pressure", rho=0.999),
     )
     local = periodic_lbm_config(initial_velocity=(1.0e-3, 0.0, 0.0))
+    # z-rows of 64 cells: the FP32/FP64 steps read the solid row bitmap (the
+    # walls at k = 0, 63 and the periodic packing reach across the z wrap)
+    chan64, chan64_solid = channel_config(nx=20, ny=12, nz=64)
+    porous64_cfg = periodic_lbm_config(nx=12, ny=10, nz=64, viscosity=0.05 / 3, force=(1.0e-5, 0.0, 0.0))
+    porous64 = porous_mask((12, 10, 64), seed=7, porosity=0.75, rmin=1.5, rmax=3.0)
     return {
         "periodic_force": dict(cfg=periodic, steps=20),
         "channel_velocity_pressure": dict(cfg=chan, steps=60, solid=chan_solid),
@@ -56,6 +61,9 @@ def _scenarios():
         "pressure_pressure": dict(cfg=pressure_both, steps=40, solid=pb_solid),
         "local_force_changed": dict(cfg=local, steps=12, force_field=_local_force(local),
                                     force_change=_change_force),
+        "channel_rows64": dict(cfg=chan64, steps=40, solid=chan64_solid),
+        "porous_rows64_force_changed": dict(cfg=porous64_cfg, steps=30, solid=porous64,
+                                            force_field=_local_force(porous64_cfg), force_change=_change_force),
     }
 
 
@@ -226,3 +234,54 @@ def test_suspension_channel_started_at_inflow_stays_bounded():
     assert 0.04 <= umax <= 0.08
     inner = u[1, 8:-8, 8:-8, 0]
     assert abs(float(inner.mean()) - 0.04) < 0.01
+
+
+def _solid_bits_expected(solid):
+    """Row bitmap of fsi_lbm_buffers.solid_bits from the mask: bit b of row
+    (i, j) is solid(i, j, (b - 1) mod nz) for b in [0, nz + 1]."""
+    nx, ny, nz = solid.shape
+    words = nz // 32 + 2
+    out = np.zeros((nx, ny, words), np.uint64)
+    for b in range(nz + 2):
+        bit = solid[:, :, (b - 1) % nz].astype(np.uint64)
+        out[:, :, b // 32] |= bit << np.uint64(b % 32)
+    return out.astype(np.uint32)
+
+
+def test_solid_row_bitmap_is_exact():
+    import fsi
+
+    cfg = periodic_lbm_config(nx=12, ny=10, nz=128)
+    solid = porous_mask((12, 10, 128), seed=9, porosity=0.7, rmin=1.0, rmax=3.0)
+    s = fsi.LBMSolver3D(cfg)
+    s.initialize(solid)
+    assert s.solid_bits is not None
+    got = s.solid_bits.cpu().numpy().view(np.uint32).reshape(12, 10, 128 // 32 + 2)
+    assert np.array_equal(got, _solid_bits_expected(solid))
+    assert fsi.LBMSolver3D(periodic_lbm_config(nx=12, ny=10, nz=96)).solid_bits is None
+    assert fsi.LBMSolver3D(dataclasses.replace(cfg, storage="fp16")).solid_bits is None
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp64"])
+@pytest.mark.parametrize("name", ["channel_rows64", "porous_rows64_force_changed"])
+def test_solid_row_bitmap_gives_the_neighbour_mask_results(name, storage):
+    """The same lattice steps reading the row bitmap or the per-cell
+    neighbour mask: bit-identical populations, density and velocity."""
+    import fsi
+
+    sc = SCENARIOS[name]
+    cfg = dataclasses.replace(sc["cfg"], storage=storage)
+    out = {}
+    for use_bits in (True, False):
+        def factory(c, use_bits=use_bits):
+            s = fsi.LBMSolver3D(c)
+            assert s.solid_bits is not None
+            if not use_bits:
+                s.solid_bits = None
+            return s
+
+        s = run_lbm(factory, cfg, sc["steps"], solid=sc.get("solid"), force_field=sc.get("force_field"),
+                    force_change=sc.get("force_change"))
+        out[use_bits] = (s.populations.cpu().numpy().copy(), s.density_numpy(), s.velocity_numpy())
+    for a, b in zip(out[True], out[False]):
+        assert np.array_equal(a, b)

@@ -154,7 +154,7 @@ def test_c_abi_library_exports_every_declared_symbol():
     assert len(names) >= 15
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.fsi_abi_version() == 4
+    assert lib.fsi_abi_version() == 5
     assert set(names) >= set(_native._SIGNATURES)
 
 
