@@ -390,7 +390,10 @@ template <typename R> __device__ __forceinline__ R wave_sum(R v) {
 template <typename S> struct PullBlock { static constexpr int value = 256; };
 template <> struct PullBlock<float> { static constexpr int value = 512; };
 
-template <typename S, int PHASE, bool FORCE_FIELD, bool FACES, bool ASM = false>
+// ROWS: nz is a multiple of 64, so every wavefront lies in one z-row
+// (blocks are multiples of 64 lanes): the row indices are wave-uniform and
+// the row offsets of the pull are scalar arithmetic
+template <typename S, int PHASE, bool FORCE_FIELD, bool FACES, bool ASM = false, bool ROWS = false>
 __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
     LbmK<typename PopStore<S>::real> c, const S* __restrict__ pin, S* __restrict__ pout,
     const uint32_t* __restrict__ nbmask, const uint32_t* __restrict__ sbits,
@@ -425,7 +428,8 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   constexpr uint32_t B = sizeof(S);
   const uint32_t nz = (uint32_t)c.nz, ny = (uint32_t)c.ny, nx = (uint32_t)c.nx;
   const uint32_t k = n % nz;
-  const uint32_t ij = n / nz;
+  uint32_t ij = n / nz;
+  if constexpr (ROWS) ij = __builtin_amdgcn_readfirstlane(ij);
   const uint32_t j = ij % ny;
   const uint32_t i = ij / ny;
   // neighbour-solid bits (bit q: the cell c + e_q is solid): from the row
@@ -435,7 +439,7 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
   // from one funnel shift), else the per-cell mask
   uint32_t mask;
   bool from_bits = false;
-  if constexpr (sizeof(S) >= 4) from_bits = sbits != nullptr;
+  if constexpr (sizeof(S) >= 4 && ROWS) from_bits = sbits != nullptr;
   if (from_bits) {
     const uint32_t words = nz / 32u + 2u;
     const uint32_t ijw = __builtin_amdgcn_readfirstlane(ij);
@@ -490,6 +494,25 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
     const uint32_t ozm = k == 0 ? nz - 1 : (uint32_t)(-1);
     const uint32_t ozp = k == nz - 1 ? (uint32_t)(-(int32_t)(nz - 1)) : 1u;
     const uint32_t all = solid ? 0u : mask;
+    if constexpr (ROWS) {
+      // source byte offset = (uniform row block + q's run) + the lane's z
+      // entry k - e_z(q), wrapped
+      const uint32_t km = k == 0 ? nz - 1 : k - 1, kp = k == nz - 1 ? 0u : k + 1;
+      const uint32_t kbyte[3] = {kp * B, k * B, km * B};  // e_z = -1, 0, +1
+      const uint32_t rowb = ij * rstride;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        uint32_t u = rowb;
+        if (ex(q) > 0) u += oxm;
+        if (ex(q) < 0) u += oxp;
+        if (ey(q) > 0) u += oym;
+        if (ey(q) < 0) u += oyp;
+        const uint32_t a0 = kbyte[ez(q) + 1] + (q * run + u * B);
+        const uint32_t a1 = opp(q) * run + own;
+        const uint32_t sel = 0u - ((all >> opp(q)) & 1u);
+        raw[q] = ld_off(pin, a0 ^ ((a0 ^ a1) & sel));
+      }
+    } else {
     const uint32_t ownel = ij * rstride + k;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
@@ -504,6 +527,7 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
       const uint32_t a1 = opp(q) * run + own;
       const uint32_t sel = 0u - ((all >> opp(q)) & 1u);
       raw[q] = ld_off(pin, a0 ^ ((a0 ^ a1) & sel));
+    }
     }
   }
   if constexpr (sizeof(S) == 2) __builtin_amdgcn_sched_barrier(0);
@@ -831,15 +855,23 @@ void launch_pull(const LbmK<typename PopStore<S>::real>& c, bool faces, const vo
                  hipStream_t st, AsmArgs<typename PopStore<S>::real> a = {}) {
   using R = typename PopStore<S>::real;
   const dim3 grid(blocks_for(c.N, PullBlock<S>::value)), block(PullBlock<S>::value);
-  const uint32_t* sbits = (b->solid_bits && sizeof(S) >= 4 && c.nz % 64 == 0) ? b->solid_bits : nullptr;
-  if (faces)
-    hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, true, ASM>), grid, block, 0, st, c, (const S*)pin,
-                       (S*)pout, b->nbmask, sbits, (const R*)force, (R*)b->rho, (R*)b->vel,
-                       (const R*)fv_prev, (R*)fv_next, a);
+  // z-rows of whole wavefronts: wave-uniform rows, and (FP32/FP64) the row
+  // bitmap of the solid flags when the caller gave one
+  const bool rows = c.nz % 64 == 0;
+  const uint32_t* sbits = (rows && b->solid_bits && sizeof(S) >= 4) ? b->solid_bits : nullptr;
+#define FSI_PULL_LAUNCH(FACES, ROWS)                                                                     \
+  hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, FACES, ASM, ROWS>), grid, block, 0, st, c, (const S*)pin, \
+                     (S*)pout, b->nbmask, sbits, (const R*)force, (R*)b->rho, (R*)b->vel,               \
+                     (const R*)fv_prev, (R*)fv_next, a)
+  if (faces && rows)
+    FSI_PULL_LAUNCH(true, true);
+  else if (faces)
+    FSI_PULL_LAUNCH(true, false);
+  else if (rows)
+    FSI_PULL_LAUNCH(false, true);
   else
-    hipLaunchKernelGGL((k_lbm_pull<S, PHASE, FF, false, ASM>), grid, block, 0, st, c, (const S*)pin,
-                       (S*)pout, b->nbmask, sbits, (const R*)force, (R*)b->rho, (R*)b->vel,
-                       (const R*)fv_prev, (R*)fv_next, a);
+    FSI_PULL_LAUNCH(false, false);
+#undef FSI_PULL_LAUNCH
 }
 
 template <typename S>
