@@ -355,8 +355,9 @@ int fsi_contact(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_cou
  * inactive particles. k: 0-2 u - v (u interpolated over the valid nodes as
  * the coupling does, coupling.py:352-372), 3-11 grad u (row-major, over the
  * valid nodes), 12 phi, 13-15 -grad(phi)/|grad(phi)| (0 if |grad phi| <
- * 1e-6), 16 the valid support weight, 17 |u - v|. No counterpart in the
- * reference. */
+ * 1e-6), 16 the valid support weight, 17 |u - v|. The lattice buffers have
+ * the MPM grid's dims (co-located grids: the closure is not offered with
+ * fsi_coupling_params.refine > 1). No counterpart in the reference. */
 int fsi_closure_features(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_buffers* cb,
                          float* X, void* stream);
 
