@@ -138,7 +138,10 @@ def test_refined_coupling_equals_the_co_located_one(storage):
     fine = _coupling_only(fsi.FSISimulation, storage, 2)
     flat = _coupling_only(fsi.FSISimulation, storage, 1)
     ref = _coupling_only(lambda c: po.OracleSimulation(c, "f64"), None, 2)
-    tol = 1e-12 if storage == "fp64" else 1e-6
+    # FP32: the two grids group the particles into different workgroups, so
+    # the float atomics that flush each workgroup's tile add in another
+    # order (a few ulp of the largest value: 1.0e-6 measured)
+    tol = 1e-12 if storage == "fp64" else 4e-6
     for key in ("cforce", "phi", "pforce", "wvalid"):
         e_flat = rel_inf(fine[key], flat[key])
         e_ref = rel_inf(fine[key], ref[key])
@@ -160,3 +163,20 @@ def test_refined_grid_rejects_the_closure():
     sim = fsi.FSISimulation(cfg)
     with pytest.raises(NotImplementedError):
         sim.coupler.set_closure(object())
+
+
+def test_refined_by_three_fp64_matches_fp64_oracle():
+    """Three MPM cells per lattice cell and axis (MPM dx = 1/3): the
+    refinement is not tied to powers of two."""
+    import fsi
+
+    kw = dict(steps=4, refine=3, substeps=24, ib=True, contact=True, wall=True)
+    ref = _run(lambda c: po.OracleSimulation(c, "f64"), None, **kw)
+    got = _run(fsi.FSISimulation, "fp64", **kw)
+    for key in FIELDS:
+        ok, err, diff = close(got[key], ref[key], 1e-9, 1e-13)
+        print(f"refine 3 fp64 {key}: rel {err:.3e} abs {diff:.3e}")
+        assert ok, (key, err, diff)
+    assert np.array_equal(got["cmask"], ref["cmask"])
+    for k in COUNTS:
+        assert got["diag"][k] == ref["diag"][k], (k, got["diag"][k], ref["diag"][k])
