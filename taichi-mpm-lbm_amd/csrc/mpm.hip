@@ -695,7 +695,6 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
   // plane split (PL = 3): partial support and interpolated velocity
   __shared__ R cpart[PL > 1 && COUPLE ? PL : 1][PL > 1 && COUPLE ? kS : 1][4];
   static_assert(PL == 1 || (PL == 3 && sizeof(R) == 4), "plane split: FP32 only");
-  static_assert(!REF || PL == 1, "refined grids: one wavefront per particle");
   const int pln = PL > 1 ? (int)threadIdx.x / kS : 0;
   const int slot = PL > 1 ? (int)threadIdx.x % kS : (int)threadIdx.x;
   const bool writer = pln == 0;  // the wavefront that writes the particle
@@ -900,6 +899,8 @@ __device__ __forceinline__ void p2g_pass(const MpmK<R>& m, const CplK<R>& cp, in
     rotate3<P>(ri, cw[0][0], cw[1][0], cw[2][0], cwir);
     rotate3<P>(rj, cw[0][1], cw[1][1], cw[2][1], cwjr);
     rotate3<P>(rk, cw[0][2], cw[1][2], cw[2][2], cwkr);
+    // plane split: this wavefront's x plane of the lattice stencil
+    if constexpr (PL > 1) cwir[0] = P(pick3(pln, cw[0][0], cw[1][0], cw[2][0]));
   } else {
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
@@ -1907,12 +1908,14 @@ int check_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
 // Plane split of the particle passes (three wavefronts per particle, one per
 // x plane of the stencil): FP32, a work list of at most split_max_chunks
 // chunks (one-batch gathers).
-// a refined MPM grid (coupling refine > 1) runs one wavefront per particle
+// An MPM grid finer than the lattice (coupling refine > 1) takes the REF
+// variants of the particle passes.
 inline bool refined(const fsi_coupling_params* c) { return c && c->refine > 1; }
 
 template <typename R>
 bool plane_split(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c) {
-  return sizeof(R) == 4 && !refined(c) && b->chunks && p->num_chunks > 0 && p->num_chunks <= p->split_max_chunks &&
+  (void)c;
+  return sizeof(R) == 4 && b->chunks && p->num_chunks > 0 && p->num_chunks <= p->split_max_chunks &&
          (!b->zero_boxes || (int64_t)Slots<3>::per_chunk * p->num_chunks <= (int64_t)b->zero_boxes_cap);
 }
 
@@ -1960,7 +1963,11 @@ int p2g_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_
   do {                                                                                                \
     if constexpr (sizeof(R) == 4) {                                                                   \
       if (split) {                                                                                    \
-        hipLaunchKernelGGL((k_p2g<R, C, P, false, 3>), grid, dim3(3 * Slots<3>::value), 0, st, FSI_P2G_ARGS); \
+        if (ref && C)                                                                                 \
+          hipLaunchKernelGGL((k_p2g<R, C, P, false, 3, true>), grid, dim3(3 * Slots<3>::value), 0, st, \
+                             FSI_P2G_ARGS);                                                           \
+        else                                                                                          \
+          hipLaunchKernelGGL((k_p2g<R, C, P, false, 3>), grid, dim3(3 * Slots<3>::value), 0, st, FSI_P2G_ARGS); \
         break;                                                                                        \
       }                                                                                               \
     }                                                                                                 \
@@ -2024,7 +2031,10 @@ int grid_g2p_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coup
   bool launched = false;
   if constexpr (sizeof(R) == 4) {
     if (plane_split<R>(p, b, c)) {
-      if (contact)
+      if (contact && refined(c))
+        hipLaunchKernelGGL((k_g2p<R, true, false, 3, true>), g2p_grid, dim3(3 * Slots<3>::value), 0, st,
+                           FSI_G2P_ARGS(true));
+      else if (contact)
         hipLaunchKernelGGL((k_g2p<R, true, false, 3>), g2p_grid, dim3(3 * Slots<3>::value), 0, st,
                            FSI_G2P_ARGS(true));
       else
@@ -2129,8 +2139,12 @@ int substeps_fused_t(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fs
   do {                                                                                                    \
     if constexpr (sizeof(R) == 4) {                                                                       \
       if (split) {                                                                                        \
-        hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false, 3>), grid, dim3(3 * Slots<3>::value), 0, st,     \
-                           FSI_FUSED_ARGS);                                                               \
+        if (ref && (CPL || CON))                                                                          \
+          hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false, 3, true>), grid, dim3(3 * Slots<3>::value), 0, \
+                             st, FSI_FUSED_ARGS);                                                         \
+        else                                                                                              \
+          hipLaunchKernelGGL((k_g2p_p2g<R, CPL, CON, false, 3>), grid, dim3(3 * Slots<3>::value), 0, st,   \
+                             FSI_FUSED_ARGS);                                                             \
         break;                                                                                            \
       }                                                                                                   \
     }                                                                                                     \

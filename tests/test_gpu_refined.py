@@ -7,9 +7,10 @@ Gates as in test_gpu_mpm_coupling_parity.py: the FP64 build matches the
 FP64 oracle to 1e-9 relative per field (float atomics reorder the sums);
 FP32 stays within twice the reference-order FP32 error band against FP64
 (+1e-6); masks and counters are exact. Each schedule the launchers pick is
-run: sorted particles with the one-batch gathers (the default at this
-size), the per-row gathers, shuffled unsorted particles (node boxes too
-large for the LDS tile: global atomics), and the two-launch substeps. The coupling
+run: sorted particles with the plane split (three wavefronts per particle,
+the default at this size; FP64 runs one wavefront per particle), the
+one-batch and the per-row gathers, shuffled unsorted particles (node boxes
+too large for the LDS tile: global atomics), and the two-launch substeps. The coupling
 alone (compute_coupling_forces) must equal that of the same particles on a
 co-located MPM grid: it reads only the lattice and the particles.
 """
@@ -32,6 +33,8 @@ COUNTS = ("unsupported_particle_count", "partial_support_particle_count", "clipp
 
 
 def _prepare(sim, schedule):
+    if schedule in ("wide", "per_row"):
+        sim.mpm.split_max_chunks = 0
     if schedule == "per_row":
         sim.mpm.wide_max_workgroups = 0
     elif schedule == "unsorted":
@@ -82,7 +85,7 @@ def oracle_runs():
             for name, kw in SCENES.items()}
 
 
-@pytest.mark.parametrize("schedule", ["wide", "per_row", "unsorted", "two_launch"])
+@pytest.mark.parametrize("schedule", ["split", "wide", "per_row", "unsorted", "two_launch"])
 @pytest.mark.parametrize("name", list(SCENES))
 def test_refined_fp64_matches_fp64_oracle(name, schedule, oracle_runs):
     import fsi
@@ -101,7 +104,7 @@ def test_refined_fp64_matches_fp64_oracle(name, schedule, oracle_runs):
         assert ref["diag"]["contact_damped_particle_count"] > 0
 
 
-@pytest.mark.parametrize("schedule", ["wide", "per_row"])
+@pytest.mark.parametrize("schedule", ["split", "wide", "per_row"])
 @pytest.mark.parametrize("name", list(SCENES))
 def test_refined_fp32_within_reference_band(name, schedule, oracle_runs):
     import fsi
