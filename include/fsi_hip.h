@@ -88,8 +88,8 @@ typedef struct fsi_lbm_buffers {
                               fraction of the 16-bit range, and a fourth slot for a measuring
                               pass (floats [6240, 8320)); each step adapts the scale from the
                               previous write's maximum. Unused otherwise (may be null) */
-  uint32_t* solid_bits;    /* optional (may be null), used when nz % 64 == 0 with FP32 or FP64
-                              storage: FSI_SOLID_BITS_WORDS(nx, ny, nz) words written by
+  uint32_t* solid_bits;    /* optional (may be null), used when nz % 64 == 0:
+                              FSI_SOLID_BITS_WORDS(nx, ny, nz) words written by
                               fsi_lbm_init, the solid flags of each z-row as bits (bit b of
                               row (i, j): cell k = b - 1, the periodic neighbours k = nz - 1 at
                               b = 0 and k = 0 at b = nz + 1). The lattice step then reads a

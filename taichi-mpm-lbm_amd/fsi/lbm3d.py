@@ -107,13 +107,13 @@ class LBMSolver3D:
                            if self.storage == "i16" else None)
         self._epoch = 0  # population writes since initialize()
 
-        # FP32/FP64 lattices whose z-rows are whole wavefronts read the solid
-        # flags as a row bitmap (scalar loads per wavefront) instead of the
-        # 4-byte neighbour mask per cell; set to None before initialize() to
-        # keep the mask
+        # lattices whose z-rows are whole wavefronts read the solid flags as
+        # a row bitmap (scalar loads per wavefront) instead of the 4-byte
+        # neighbour mask per cell; set to None before initialize() to keep
+        # the mask
         self.solid_bits = (torch.zeros(nat.solid_bits_words(self.nx, self.ny, self.nz), dtype=torch.int32,
                                        device=dev)
-                           if self.storage in ("fp32", "fp64") and self.nz % 64 == 0 else None)
+                           if self.nz % 64 == 0 else None)
 
         self._params = self._make_params()
         self._buffers = self._make_buffers()

@@ -259,14 +259,17 @@ def test_solid_row_bitmap_is_exact():
     got = s.solid_bits.cpu().numpy().view(np.uint32).reshape(12, 10, 128 // 32 + 2)
     assert np.array_equal(got, _solid_bits_expected(solid))
     assert fsi.LBMSolver3D(periodic_lbm_config(nx=12, ny=10, nz=96)).solid_bits is None
-    assert fsi.LBMSolver3D(dataclasses.replace(cfg, storage="fp16")).solid_bits is None
+    for storage in ("fp16", "i16", "fp64"):
+        assert fsi.LBMSolver3D(dataclasses.replace(cfg, storage=storage)).solid_bits is not None
 
 
-@pytest.mark.parametrize("storage", ["fp32", "fp64"])
+@pytest.mark.parametrize("storage", ["fp32", "fp64", "fp16", "i16"])
 @pytest.mark.parametrize("name", ["channel_rows64", "porous_rows64_force_changed"])
 def test_solid_row_bitmap_gives_the_neighbour_mask_results(name, storage):
     """The same lattice steps reading the row bitmap or the per-cell
-    neighbour mask: bit-identical populations, density and velocity."""
+    neighbour mask: bit-identical populations, density and velocity (the
+    16-bit storages extract each bounce-back select from the rows' bits,
+    FP32/FP64 assemble the mask from them)."""
     import fsi
 
     sc = SCENARIOS[name]
