@@ -518,7 +518,7 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
     const uint32_t ozm = k == 0 ? nz - 1 : (uint32_t)(-1);
     const uint32_t ozp = k == nz - 1 ? (uint32_t)(-(int32_t)(nz - 1)) : 1u;
     // all ones when the source c - e_q = c + e_opp(q) of a fluid cell is
-    // solid (its bit sign-extended: one bit-field extract per population)
+    // solid (BITS: its row bit sign-extended by one bit-field extract)
     const uint32_t all = solid ? 0u : mask;
     uint32_t winf[3][3];
     if constexpr (BITS) {
@@ -553,21 +553,21 @@ __global__ __launch_bounds__(PullBlock<S>::value) void k_lbm_pull(
         raw[q] = ld_off(pin, a0 ^ ((a0 ^ a1) & sel));
       }
     } else {
-    const uint32_t ownel = ij * rstride + k;
+      const uint32_t ownel = ij * rstride + k;
 #pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      uint32_t src = ownel;  // c - e_q
-      if (ex(q) > 0) src += oxm;
-      if (ex(q) < 0) src += oxp;
-      if (ey(q) > 0) src += oym;
-      if (ey(q) < 0) src += oyp;
-      if (ez(q) > 0) src += ozm;
-      if (ez(q) < 0) src += ozp;
-      const uint32_t a0 = q * run + src * B;
-      const uint32_t a1 = opp(q) * run + own;
-      const uint32_t sel = bounce(q);
-      raw[q] = ld_off(pin, a0 ^ ((a0 ^ a1) & sel));
-    }
+      for (int q = 0; q < 19; ++q) {
+        uint32_t src = ownel;  // c - e_q
+        if (ex(q) > 0) src += oxm;
+        if (ex(q) < 0) src += oxp;
+        if (ey(q) > 0) src += oym;
+        if (ey(q) < 0) src += oyp;
+        if (ez(q) > 0) src += ozm;
+        if (ez(q) < 0) src += ozp;
+        const uint32_t a0 = q * run + src * B;
+        const uint32_t a1 = opp(q) * run + own;
+        const uint32_t sel = bounce(q);
+        raw[q] = ld_off(pin, a0 ^ ((a0 ^ a1) & sel));
+      }
     }
   }
   if constexpr (sizeof(S) == 2) __builtin_amdgcn_sched_barrier(0);
