@@ -177,6 +177,9 @@ def _run_sim(make, storage, steps, **kw):
 COUPLED = {
     "penalty": dict(steps=6),
     "penalty_ib_contact": dict(steps=4, ib=True, contact=True),
+    # z-rows of 64 cells: the lattice passes of the coupled step (the collide
+    # with the force assembly among them) read the solid row bitmap
+    "ib_contact_rows64": dict(steps=4, ib=True, contact=True, n=(20, 12, 64)),
 }
 
 
