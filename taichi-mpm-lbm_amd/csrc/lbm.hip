@@ -385,10 +385,11 @@ template <typename R> __device__ __forceinline__ R wave_sum(R v) {
 // FACES: some x face is not periodic. Requires 19 * N * sizeof(S) < 2^32.
 // Workgroup size of the lattice step. A workgroup reads 19 contiguous runs
 // whose z-shifted ones straddle one extra cache line at its ends; 1024 lanes
-// amortise that for FP32 (measured 2% faster at 256^3), while FP16 runs best
-// at 256.
+// amortise that for FP32 (measured at 256^3 with the row bitmap: 423 us
+// against 430 us at 512 and 437 us at 256), while the 16-bit storages run
+// best at 256 (FP16 215 us against 226 us at 512 and 237 us at 128).
 template <typename S> struct PullBlock { static constexpr int value = 256; };
-template <> struct PullBlock<float> { static constexpr int value = 512; };
+template <> struct PullBlock<float> { static constexpr int value = 1024; };
 
 // ROWS: nz is a multiple of 64, so every wavefront lies in one z-row
 // (blocks are multiples of 64 lanes): the row indices are wave-uniform and
