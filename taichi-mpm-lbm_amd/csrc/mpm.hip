@@ -1905,16 +1905,15 @@ int check_mpm(const fsi_mpm_params* p, const fsi_mpm_buffers* b) {
   return 0;
 }
 
-// Plane split of the particle passes (three wavefronts per particle, one per
-// x plane of the stencil): FP32, a work list of at most split_max_chunks
-// chunks (one-batch gathers).
 // An MPM grid finer than the lattice (coupling refine > 1) takes the REF
 // variants of the particle passes.
 inline bool refined(const fsi_coupling_params* c) { return c && c->refine > 1; }
 
+// Plane split of the particle passes (three wavefronts per particle, one per
+// x plane of the stencil, of the lattice stencil too on a refined grid):
+// FP32, a work list of at most split_max_chunks chunks (one-batch gathers).
 template <typename R>
-bool plane_split(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params* c) {
-  (void)c;
+bool plane_split(const fsi_mpm_params* p, const fsi_mpm_buffers* b, const fsi_coupling_params*) {
   return sizeof(R) == 4 && b->chunks && p->num_chunks > 0 && p->num_chunks <= p->split_max_chunks &&
          (!b->zero_boxes || (int64_t)Slots<3>::per_chunk * p->num_chunks <= (int64_t)b->zero_boxes_cap);
 }
