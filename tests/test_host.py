@@ -181,3 +181,45 @@ def test_bench_command_line_on_the_cpu():
     assert r.returncode == 0 and "--large-error-steps" in r.stdout
     r = subprocess.run([sys.executable, str(bench), "--gpus", "8"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_bench_scenes_as_the_configurations_state_them():
+    """The coupled scenes bench.py measures: the 32^3 cube and the 16-sphere
+    suspension at dx_mpm = dx_lbm / 2 (8 particles per MPM cell), their
+    co-located counterparts, and the elastic P-wave Courant number 0.3 that
+    sets their substep counts."""
+    import numpy as np
+
+    import bench
+    from fsi.scenes import body_moduli, sphere_pack
+
+    def courant(cfg, E):
+        nu, rho = cfg.mpm.poisson_ratio, cfg.mpm.density
+        c_p = np.sqrt(E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) / rho)
+        return c_p * (cfg.lbm_dt / cfg.coupling.mpm_substeps_per_lbm_step) / cfg.mpm.dx
+
+    for refine, lo, hi, sub in ((2, (10, 14, 14), (14, 18, 18), 224), (1, (8, 12, 12), (16, 20, 20), 112)):
+        cfg, solid, pos = bench.coupled_scene_32(refine=refine)
+        cfg.validate()
+        assert len(pos) == 4096 and cfg.coupling.mpm_refinement == refine
+        assert (cfg.mpm.nx, cfg.mpm.dx) == (32 * refine, 1.0 / refine)
+        assert cfg.coupling.mpm_substeps_per_lbm_step == sub
+        assert np.all(pos >= np.array(lo)) and np.all(pos <= np.array(hi))
+        assert np.allclose(pos.mean(axis=0), (12.0, 16.0, 16.0), atol=0.05)
+        assert bench.particle_volume(cfg) == (0.5 / refine) ** 3
+        assert 0.25 < courant(cfg, cfg.mpm.youngs_modulus) <= 0.3
+
+    fine = bench.coupled_scene_large(refine=2)
+    flat = bench.coupled_scene_large(refine=1)
+    fine[0].validate()
+    assert len(fine[2]) == 3591333 and len(flat[2]) == 448861
+    assert (fine[0].mpm.nx, fine[0].mpm.ny, fine[0].mpm.nz) == (512, 256, 256)
+    # the same 16 spheres, 8x the particles, the same per-sphere moduli
+    _, info2 = sphere_pack((256, 128, 128), 16, 6.0, 12.0, spacing=0.25, seed=0)
+    _, info1 = sphere_pack((256, 128, 128), 16, 6.0, 12.0, spacing=0.5, seed=0)
+    assert np.array_equal(info1["centres"], info2["centres"]) and np.array_equal(info1["radii"], info2["radii"])
+    assert np.array_equal(body_moduli(info2["counts"], 5.0e2, 5.0e3, seed=0)[1],
+                          body_moduli(info1["counts"], 5.0e2, 5.0e3, seed=0)[1])
+    assert np.isclose(fine[3].max(), flat[3].max())
+    for scene in (fine, flat):
+        assert 0.25 < courant(scene[0], scene[3].max()) <= 0.3
